@@ -1,0 +1,121 @@
+"""ctypes binding of the C ABI in include/fslbm.h (libfslbm.so).
+
+The library is loaded from the package directory; a missing library is an
+error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libfslbm.so"
+
+F64, F32, F16S, BF16S = 0, 1, 2, 3
+PRECISIONS = {"f64": F64, "f32": F32, "f16s": F16S, "bf16s": BF16S}
+KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TILED = 0, 1, 2
+
+# Every symbol include/fslbm.h declares (checked by the CPU tests).
+EXPORTS = [
+    "fslbm_abi_version", "fslbm_error_string", "fslbm_create", "fslbm_destroy",
+    "fslbm_last_error", "fslbm_set_params", "fslbm_get_params",
+    "fslbm_set_kernel", "fslbm_get_kernel", "fslbm_set_state",
+    "fslbm_get_state", "fslbm_set_state_f32", "fslbm_get_state_f32",
+    "fslbm_step", "fslbm_set_gauge", "fslbm_monitor", "fslbm_reduce",
+    "fslbm_probe", "fslbm_get_plane", "fslbm_stats_config",
+    "fslbm_stats_accumulate", "fslbm_stats_read", "fslbm_stats_load",
+    "fslbm_stats_points", "fslbm_moments_reset", "fslbm_moments_accumulate",
+    "fslbm_moments_read", "fslbm_sync", "fslbm_time_steps",
+    "fslbm_device_bytes", "fslbm_pipe_m_mask", "fslbm_circle_mask",
+    "fslbm_init_poiseuille", "fslbm_init_taylor_green", "fslbm_fold_mask",
+]
+
+
+class Params(C.Structure):
+    """fslbm_params: the six values of param.txt (pipe.m:67-69)."""
+
+    _fields_ = [
+        ("tblend", C.c_double),
+        ("pin", C.c_double),
+        ("nu", C.c_double),
+        ("blend", C.c_double),
+        ("d1", C.c_int),
+        ("delta", C.c_int),
+    ]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libfslbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, ip, i, d = C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_double
+    sig = {
+        "fslbm_abi_version": (i, []),
+        "fslbm_error_string": (C.c_char_p, [i]),
+        "fslbm_create": (i, [C.POINTER(Params), C.c_void_p, i, i, i, i, C.POINTER(vp)]),
+        "fslbm_destroy": (None, [vp]),
+        "fslbm_last_error": (C.c_char_p, [vp]),
+        "fslbm_set_params": (i, [vp, C.POINTER(Params)]),
+        "fslbm_get_params": (i, [vp, C.POINTER(Params)]),
+        "fslbm_set_kernel": (i, [vp, i]),
+        "fslbm_get_kernel": (i, [vp]),
+        "fslbm_set_state": (i, [vp, _dp, _dp, _dp, _dp]),
+        "fslbm_get_state": (i, [vp, _dp, _dp, _dp, _dp]),
+        "fslbm_set_state_f32": (i, [vp, _fp, _fp, _fp, _fp]),
+        "fslbm_get_state_f32": (i, [vp, _fp, _fp, _fp, _fp]),
+        "fslbm_step": (i, [vp, i]),
+        "fslbm_set_gauge": (i, [vp, d]),
+        "fslbm_monitor": (i, [vp, i, _dp]),
+        "fslbm_reduce": (i, [vp, _dp]),
+        "fslbm_probe": (i, [vp, i, i, i, _dp]),
+        "fslbm_get_plane": (i, [vp, i, i, i, i, _dp]),
+        "fslbm_stats_config": (i, [vp, i, i]),
+        "fslbm_stats_accumulate": (i, [vp, i]),
+        "fslbm_stats_read": (i, [vp, C.c_void_p, ip, ip]),
+        "fslbm_stats_load": (i, [vp, _dp, i, i]),
+        "fslbm_stats_points": (i, [vp]),
+        "fslbm_moments_reset": (i, [vp]),
+        "fslbm_moments_accumulate": (i, [vp]),
+        "fslbm_moments_read": (i, [vp, C.c_void_p, C.POINTER(C.c_long)]),
+        "fslbm_sync": (i, [vp]),
+        "fslbm_time_steps": (i, [vp, i, C.POINTER(d)]),
+        "fslbm_device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
+        "fslbm_pipe_m_mask": (C.c_long, [i, i, i, _u8]),
+        "fslbm_circle_mask": (C.c_long, [i, i, i, d, _u8]),
+        "fslbm_init_poiseuille": (i, [i, i, i, _u8, d, d, d, d, C.c_uint64, _dp, _dp, _dp, _dp]),
+        "fslbm_init_taylor_green": (i, [i, i, i, d, d, _dp, _dp, _dp, _dp]),
+        "fslbm_fold_mask": (i, [i, i, i, i, _u8, _u8]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class FslbmError(RuntimeError):
+    pass
+
+
+def check(rc: int, ctx=None) -> int:
+    if rc < 0:
+        L = lib()
+        msg = L.fslbm_last_error(ctx).decode() if ctx is not None else \
+            L.fslbm_last_error(None).decode()
+        raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}")
+    return rc

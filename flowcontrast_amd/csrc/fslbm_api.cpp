@@ -1,0 +1,748 @@
+// Host side of the C ABI declared in include/fslbm.h.
+//
+// One context owns: two device state buffers (ping-pong, four SoA planes
+// each), the device mask, FP64 reduction scratch, the statistics
+// accumulators and a HIP stream.  The step sequence of the reference
+// (38 FSLBMcalc slabs, 310 shiftback copies and bcond per step,
+// FSLBM.cu:622-628) becomes one kernel launch per step.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fslbm_internal.h"
+#include "fslbm_store.h"
+
+using namespace fslbm;
+
+struct fslbm_ctx {
+    Geo g{};
+    int precision = FSLBM_F32;
+    int kernel = FSLBM_KERNEL_AUTO;
+    int device = 0;
+    fslbm_params prm{};
+    double gauge = 0.0;        // pending p2 for the next step
+    Quant q{1.f, 1.f, 1.f, 1.f, 0.f, 0.f};
+    void *state[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t *mask = nullptr;   // device mask (nullptr: periodic box)
+    std::vector<uint8_t> hmask;
+    double *partials = nullptr;  // reduction scratch
+    int npart = 1024;
+    double *red = nullptr;       // 8 doubles of results
+    double *hred = nullptr;      // pinned host copy
+    // reference plane statistics
+    double *stats = nullptr;
+    int stats_n = 0, stats_x = -1, stats_d2 = 0, cnt = 0, cnt2 = 0;
+    // moments
+    double *mom = nullptr;
+    long mom_samples = 0;
+    double *plane = nullptr;     // plane extraction buffer
+    size_t plane_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t dev_bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(fslbm_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                    \
+    do {                                                                   \
+        hipError_t e_ = (expr);                                            \
+        if (e_ != hipSuccess)                                              \
+            return fail((c), FSLBM_EHIP, "%s failed: %s", #expr,           \
+                        hipGetErrorString(e_));                            \
+    } while (0)
+
+thread_local std::string g_create_err;
+
+int check_params(fslbm_ctx *c, const fslbm_params *p)
+{
+    if (!p) return fail(c, FSLBM_EINVAL, "params is NULL");
+    if (p->d1 < 1) return fail(c, FSLBM_EINVAL, "d1 must be >= 1 (got %d)", p->d1);
+    if (!std::isfinite(p->tblend) || !std::isfinite(p->pin) ||
+        !std::isfinite(p->nu) || !std::isfinite(p->blend))
+        return fail(c, FSLBM_EINVAL, "non-finite run parameter");
+    if (p->delta < 0 || p->delta > 2)
+        return fail(c, FSLBM_EINVAL, "delta must be 0, 1 or 2 (got %d)", p->delta);
+    const Geo &g = c->g;
+    // The corrector reaches 2*d1 cells; a single wrap must stay inside.
+    if (2 * p->d1 > g.nz)
+        return fail(c, FSLBM_EINVAL, "2*d1 = %d exceeds nz = %d", 2 * p->d1, g.nz);
+    if (g.periodic && (2 * p->d1 > g.nx || 2 * p->d1 > g.ny))
+        return fail(c, FSLBM_EINVAL, "2*d1 = %d exceeds the periodic box", 2 * p->d1);
+    return FSLBM_OK;
+}
+
+template <typename T>
+void encode(const fslbm_ctx *c, const T *const src[4], void *dst)
+{
+    const long long N = c->g.N;
+    const int nxy = c->g.nx * c->g.ny;
+    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
+    auto run = [&](auto tag) {
+        using ST = decltype(tag);
+        using S = typename ST::S;
+        S *d = (S *)dst;
+        for (long long i = 0; i < N; i++) {
+            const bool wall = m && m[i] != 0;
+            const int z = (int)(i / nxy);
+            d[i] = wall ? S(0) : ST::st_p((typename ST::R)src[0][i], z, c->q);
+            for (int f = 1; f < 4; f++)
+                d[f * N + i] = wall ? S(0) : ST::st_u((typename ST::R)src[f][i], c->q);
+        }
+    };
+    switch (c->precision) {
+    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
+    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
+    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
+    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
+    }
+}
+
+template <typename T>
+void decode(const fslbm_ctx *c, const void *src, T *const dst[4])
+{
+    const long long N = c->g.N;
+    const int nxy = c->g.nx * c->g.ny;
+    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
+    auto run = [&](auto tag) {
+        using ST = decltype(tag);
+        using S = typename ST::S;
+        const S *s = (const S *)src;
+        for (long long i = 0; i < N; i++) {
+            const bool wall = m && m[i] != 0;
+            const int z = (int)(i / nxy);
+            dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, c->q);
+            for (int f = 1; f < 4; f++)
+                dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], c->q);
+        }
+    };
+    switch (c->precision) {
+    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
+    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
+    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
+    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
+    }
+}
+
+// Smallest power of two >= v (v > 0), as a float.
+float pow2_at_least(double v)
+{
+    int e;
+    double m = std::frexp(v, &e);  // v = m * 2^e, m in [0.5, 1)
+    if (m == 0.5) e -= 1;
+    return std::ldexp(1.0f, e);
+}
+
+// Choose the shifted-storage reference and scales from the state: the
+// pressure reference is the mean plus the mean axial gradient -pin/nz;
+// scales map the largest magnitude to [512, 1024) (exact powers of two).
+template <typename T>
+void choose_quant(fslbm_ctx *c, const T *const src[4])
+{
+    const long long N = c->g.N;
+    const int nxy = c->g.nx * c->g.ny;
+    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
+    const double slope = c->g.periodic ? 0.0 : -c->prm.pin / c->g.nz;
+    double sum = 0;
+    long long n = 0;
+    for (long long i = 0; i < N; i++) {
+        if (m && m[i] != 0) continue;
+        sum += (double)src[0][i] - slope * (double)(i / nxy);
+        n++;
+    }
+    const double base = n ? sum / (double)n : 0.0;
+    double pmax = 0, umax = 0;
+    for (long long i = 0; i < N; i++) {
+        if (m && m[i] != 0) continue;
+        double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
+        pmax = std::max(pmax, std::fabs(dp));
+        for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
+    }
+    // Headroom for growth: at least 1e-3 for velocity, 1e-4 for pressure.
+    umax = std::max(umax, 1e-3);
+    pmax = std::max(pmax, std::max(1e-4, std::fabs(c->prm.pin)));
+    const float su = pow2_at_least(umax) / 1024.0f;
+    const float sp = pow2_at_least(pmax) / 1024.0f;
+    c->q.su = su;
+    c->q.inv_su = 1.0f / su;
+    c->q.sp = sp;
+    c->q.inv_sp = 1.0f / sp;
+    c->q.pbase = (float)base;
+    c->q.pslope = (float)slope;
+}
+
+template <typename T>
+int set_state_impl(fslbm_ctx *c, const T *p, const T *ux, const T *uy,
+                   const T *uz)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!p || !ux || !uy || !uz) return fail(c, FSLBM_EINVAL, "NULL state array");
+    const T *src[4] = {p, ux, uy, uz};
+    if (c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S)
+        choose_quant(c, src);
+    const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
+    void *host = nullptr;
+    HIPCHK(c, hipHostMalloc(&host, bytes, hipHostMallocDefault));
+    encode(c, src, host);
+    hipError_t e = hipMemcpyAsync(c->state[c->cur], host, bytes,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->state[c->cur ^ 1], host, bytes,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipHostFree(host);
+    if (e != hipSuccess)
+        return fail(c, FSLBM_EHIP, "state upload failed: %s", hipGetErrorString(e));
+    c->gauge = 0.0;
+    return FSLBM_OK;
+}
+
+template <typename T>
+int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!p || !ux || !uy || !uz) return fail(c, FSLBM_EINVAL, "NULL state array");
+    const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
+    void *host = nullptr;
+    HIPCHK(c, hipHostMalloc(&host, bytes, hipHostMallocDefault));
+    hipError_t e = hipMemcpyAsync(host, c->state[c->cur], bytes,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {
+        T *dst[4] = {p, ux, uy, uz};
+        decode(c, host, dst);
+    }
+    (void)hipHostFree(host);
+    if (e != hipSuccess)
+        return fail(c, FSLBM_EHIP, "state download failed: %s", hipGetErrorString(e));
+    return FSLBM_OK;
+}
+
+bool use_tiled(const fslbm_ctx *c)
+{
+    if (c->kernel == FSLBM_KERNEL_PLAIN) return false;
+    return tiled_supported(c->g, c->precision, c->prm.d1);
+}
+
+int launch_one_step(fslbm_ctx *c, double gauge)
+{
+    StepArgs a;
+    a.tblend = c->prm.tblend;
+    a.pin = c->prm.pin;
+    a.nu = c->prm.nu;
+    a.blend = c->prm.blend;
+    a.d1 = c->prm.d1;
+    a.q = c->q;
+    const bool shifted = c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S;
+    // Shifted storage applies the gauge to the pressure reference: the
+    // stored deviations are unchanged and later loads see p - gauge.
+    a.gauge = shifted ? 0.0 : gauge;
+    const void *in = c->state[c->cur];
+    void *out = c->state[c->cur ^ 1];
+    hipError_t e;
+    if (c->kernel == FSLBM_KERNEL_TILED && !tiled_supported(c->g, c->precision, c->prm.d1))
+        return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
+    if (use_tiled(c))
+        e = launch_step_tiled(c->g, c->precision, in, out, c->mask, a, c->stream);
+    else
+        e = launch_step_plain(c->g, c->precision, in, out, c->mask, a, c->stream);
+    if (e != hipSuccess)
+        return fail(c, FSLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
+    c->cur ^= 1;
+    if (shifted && gauge != 0.0) c->q.pbase = (float)((double)c->q.pbase - gauge);
+    return FSLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fslbm_abi_version(void) { return FSLBM_ABI_VERSION; }
+
+const char *fslbm_error_string(int code)
+{
+    switch (code) {
+    case FSLBM_OK: return "ok";
+    case FSLBM_EINVAL: return "invalid argument";
+    case FSLBM_ENOMEM: return "out of memory";
+    case FSLBM_EHIP: return "HIP runtime error";
+    case FSLBM_ENODEV: return "no usable gfx950 device";
+    case FSLBM_ESTATE: return "invalid state for this call";
+    }
+    return "unknown error";
+}
+
+const char *fslbm_last_error(fslbm_t c)
+{
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+void fslbm_destroy(fslbm_t c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->state) if (p) (void)hipFree(p);
+    if (c->mask) (void)hipFree(c->mask);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->red) (void)hipFree(c->red);
+    if (c->hred) (void)hipHostFree(c->hred);
+    if (c->stats) (void)hipFree(c->stats);
+    if (c->mom) (void)hipFree(c->mom);
+    if (c->plane) (void)hipFree(c->plane);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
+                 int nz, int precision, fslbm_t *out)
+{
+    g_create_err.clear();
+    if (!out) { g_create_err = "out is NULL"; return FSLBM_EINVAL; }
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) {
+        g_create_err = "box dimensions must be positive";
+        return FSLBM_EINVAL;
+    }
+    const long long N = (long long)nx * ny * nz;
+    if (N * 4 >= (1ll << 31)) {
+        g_create_err = "domain too large for 32-bit plane indexing (4*N >= 2^31)";
+        return FSLBM_EINVAL;
+    }
+    if (precision < FSLBM_F64 || precision > FSLBM_BF16S) {
+        g_create_err = "unknown precision mode";
+        return FSLBM_EINVAL;
+    }
+    fslbm_ctx *c = new fslbm_ctx();
+    c->g.nx = nx; c->g.ny = ny; c->g.nz = nz; c->g.N = N;
+    c->g.periodic = mask == nullptr;
+    c->precision = precision;
+    int rc = check_params(c, prm);
+    if (rc) { g_create_err = c->err; delete c; return rc; }
+    c->prm = *prm;
+    if (mask) {
+        const long long nxy = (long long)nx * ny;
+        for (long long i = 0; i < N; i++) {
+            const uint8_t m = mask[i];
+            if (m != 0 && m != 125) {
+                g_create_err = "mask codes must be 0 (fluid) or 125 (wall); code " +
+                               std::to_string(m) + " at cell " + std::to_string(i);
+                delete c;
+                return FSLBM_EINVAL;
+            }
+            if (m == 0) {
+                const int x = (int)(i % nx), y = (int)((i / nx) % ny);
+                if (x == 0 || y == 0 || x == nx - 1 || y == ny - 1) {
+                    (void)nxy;
+                    g_create_err = "fluid cell on the x/y face of the box at (" +
+                                   std::to_string(x) + "," + std::to_string(y) + "," +
+                                   std::to_string(i / nxy) + "); a wall frame is required";
+                    delete c;
+                    return FSLBM_EINVAL;
+                }
+            }
+        }
+        c->hmask.assign(mask, mask + N);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_err = "no HIP device visible";
+        delete c;
+        return FSLBM_ENODEV;
+    }
+    (void)hipGetDevice(&c->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_err = std::string("device is not gfx950: ") + prop.gcnArchName;
+        delete c;
+        return FSLBM_ENODEV;
+    }
+    const size_t sbytes = storage_bytes(precision) * 4 * (size_t)N;
+    auto alloc = [&](void **p, size_t b) -> bool {
+        if (hipMalloc(p, b) != hipSuccess) return false;
+        c->dev_bytes += b;
+        return true;
+    };
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->ev0) == hipSuccess &&
+              hipEventCreate(&c->ev1) == hipSuccess &&
+              alloc(&c->state[0], sbytes) && alloc(&c->state[1], sbytes) &&
+              alloc((void **)&c->partials, sizeof(double) * 8 * c->npart) &&
+              alloc((void **)&c->red, sizeof(double) * 8) &&
+              hipHostMalloc((void **)&c->hred, sizeof(double) * 8, hipHostMallocDefault) == hipSuccess;
+    if (ok && mask) {
+        ok = alloc((void **)&c->mask, (size_t)N) &&
+             hipMemcpy(c->mask, mask, (size_t)N, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (ok) {
+        ok = hipMemsetAsync(c->state[0], 0, sbytes, c->stream) == hipSuccess &&
+             hipMemsetAsync(c->state[1], 0, sbytes, c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    if (!ok) {
+        g_create_err = "device allocation failed (" + std::to_string(sbytes * 2) +
+                       " bytes of state requested)";
+        fslbm_destroy(c);
+        return FSLBM_ENOMEM;
+    }
+    if (precision == FSLBM_F16S || precision == FSLBM_BF16S) {
+        // Zero state: reference 0, unit scales until a state is uploaded.
+        c->q = Quant{1.0f / 1024, 1024.0f, 1.0f / 1024, 1024.0f, 0.0f,
+                     c->g.periodic ? 0.0f : (float)(-prm->pin / nz)};
+    }
+    *out = c;
+    return FSLBM_OK;
+}
+
+int fslbm_set_params(fslbm_t c, const fslbm_params *prm)
+{
+    if (!c) return FSLBM_EINVAL;
+    int rc = check_params(c, prm);
+    if (rc) return rc;
+    c->prm = *prm;
+    return FSLBM_OK;
+}
+
+int fslbm_get_params(fslbm_t c, fslbm_params *prm)
+{
+    if (!c || !prm) return FSLBM_EINVAL;
+    *prm = c->prm;
+    return FSLBM_OK;
+}
+
+int fslbm_set_kernel(fslbm_t c, int variant)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (variant < FSLBM_KERNEL_AUTO || variant > FSLBM_KERNEL_TILED)
+        return fail(c, FSLBM_EINVAL, "unknown kernel variant %d", variant);
+    if (variant == FSLBM_KERNEL_TILED && !tiled_supported(c->g, c->precision, c->prm.d1))
+        return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
+    c->kernel = variant;
+    return FSLBM_OK;
+}
+
+int fslbm_get_kernel(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    return use_tiled(c) ? FSLBM_KERNEL_TILED : FSLBM_KERNEL_PLAIN;
+}
+
+int fslbm_set_state(fslbm_t c, const double *p, const double *ux,
+                    const double *uy, const double *uz)
+{
+    return set_state_impl(c, p, ux, uy, uz);
+}
+
+int fslbm_set_state_f32(fslbm_t c, const float *p, const float *ux,
+                        const float *uy, const float *uz)
+{
+    return set_state_impl(c, p, ux, uy, uz);
+}
+
+int fslbm_get_state(fslbm_t c, double *p, double *ux, double *uy, double *uz)
+{
+    return get_state_impl(c, p, ux, uy, uz);
+}
+
+int fslbm_get_state_f32(fslbm_t c, float *p, float *ux, float *uy, float *uz)
+{
+    return get_state_impl(c, p, ux, uy, uz);
+}
+
+int fslbm_step(fslbm_t c, int nsteps)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (nsteps < 0) return fail(c, FSLBM_EINVAL, "nsteps < 0");
+    for (int s = 0; s < nsteps; s++) {
+        int rc = launch_one_step(c, c->gauge);
+        if (rc) return rc;
+        c->gauge = 0.0;  // bcond(..., p2); p2 = 0  (FSLBM.cu:628)
+    }
+    return FSLBM_OK;
+}
+
+int fslbm_set_gauge(fslbm_t c, double p2)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!std::isfinite(p2)) return fail(c, FSLBM_EINVAL, "non-finite gauge");
+    c->gauge = p2;
+    return FSLBM_OK;
+}
+
+int fslbm_monitor(fslbm_t c, int set_gauge, double out[4])
+{
+    if (!c || !out) return FSLBM_EINVAL;
+    HIPCHK(c, launch_monitor(c->g, c->precision, c->state[c->cur], c->mask, c->q,
+                             c->partials, c->npart, c->red, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hred, c->red, sizeof(double) * 3,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double l1 = c->hred[0];
+    if (l1 <= 0) return fail(c, FSLBM_ESTATE, "no fluid cell in plane z=0");
+    // The outlet plane is the periodic image of plane 0 with p - pin.
+    const double q1 = c->hred[2] / l1;
+    const double p1 = c->hred[1] / l1;
+    const double p2 = (c->hred[1] - l1 * c->prm.pin) / l1;
+    out[0] = q1;
+    out[1] = q1;
+    out[2] = p1;
+    out[3] = p2;
+    if (set_gauge) c->gauge = p2;
+    return FSLBM_OK;
+}
+
+int fslbm_reduce(fslbm_t c, double out[8])
+{
+    if (!c || !out) return FSLBM_EINVAL;
+    HIPCHK(c, launch_reduce(c->g, c->precision, c->state[c->cur], c->mask, c->q,
+                            c->partials, c->npart, c->red, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hred, c->red, sizeof(double) * 8,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->hred, sizeof(double) * 8);
+    return FSLBM_OK;
+}
+
+static int ensure_plane(fslbm_ctx *c, size_t n)
+{
+    if (c->plane_cap >= n) return FSLBM_OK;
+    if (c->plane) { (void)hipFree(c->plane); c->dev_bytes -= c->plane_cap * sizeof(double); }
+    c->plane = nullptr;
+    c->plane_cap = 0;
+    if (hipMalloc((void **)&c->plane, n * sizeof(double)) != hipSuccess)
+        return fail(c, FSLBM_ENOMEM, "plane buffer allocation failed");
+    c->plane_cap = n;
+    c->dev_bytes += n * sizeof(double);
+    return FSLBM_OK;
+}
+
+int fslbm_get_plane(fslbm_t c, int axis, int idx, int d2, int field,
+                    double *out)
+{
+    if (!c || !out) return FSLBM_EINVAL;
+    if (field < 0 || field > 3) return fail(c, FSLBM_EINVAL, "field must be 0..3");
+    if (d2 < 0 || d2 > c->g.nz) return fail(c, FSLBM_EINVAL, "d2 out of range");
+    size_t n;
+    if (axis == 0) {
+        if (idx < 0 || idx >= c->g.nx) return fail(c, FSLBM_EINVAL, "x plane out of range");
+        n = (size_t)c->g.ny * (c->g.nz + 2 * d2);
+    } else if (axis == 2) {
+        if (idx < 0 || idx >= c->g.nz + 2 * d2) return fail(c, FSLBM_EINVAL, "z plane out of range");
+        n = (size_t)c->g.nx * c->g.ny;
+    } else {
+        return fail(c, FSLBM_EINVAL, "axis must be 0 (YZ) or 2 (XY)");
+    }
+    int rc = ensure_plane(c, n);
+    if (rc) return rc;
+    HIPCHK(c, launch_plane(c->g, c->precision, c->state[c->cur], c->mask, c->q,
+                           c->prm.pin, axis, idx, d2, field, c->plane, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, c->plane, n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSLBM_OK;
+}
+
+int fslbm_probe(fslbm_t c, int x, int y, int z, double out[4])
+{
+    if (!c || !out) return FSLBM_EINVAL;
+    if (x < 0 || y < 0 || z < 0 || x >= c->g.nx || y >= c->g.ny || z >= c->g.nz)
+        return fail(c, FSLBM_EINVAL, "probe outside the box");
+    const size_t sb = storage_bytes(c->precision);
+    const long long i = x + (long long)c->g.nx * (y + (long long)c->g.ny * z);
+    unsigned char raw[4][8];
+    for (int f = 0; f < 4; f++)
+        HIPCHK(c, hipMemcpyAsync(raw[f], (const char *)c->state[c->cur] +
+                                             sb * (f * c->g.N + i),
+                                 sb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool wall = !c->g.periodic && c->hmask[i] != 0;
+    auto dec = [&](auto tag, int f) -> double {
+        using ST = decltype(tag);
+        typename ST::S v;
+        std::memcpy(&v, raw[f], sizeof v);
+        return f == 0 ? (double)ST::ld_p(v, z, c->q) : (double)ST::ld_u(v, c->q);
+    };
+    for (int f = 0; f < 4; f++) {
+        double v = 0;
+        switch (c->precision) {
+        case FSLBM_F64: v = dec(Store<FSLBM_F64>{}, f); break;
+        case FSLBM_F32: v = dec(Store<FSLBM_F32>{}, f); break;
+        case FSLBM_F16S: v = dec(Store<FSLBM_F16S>{}, f); break;
+        case FSLBM_BF16S: v = dec(Store<FSLBM_BF16S>{}, f); break;
+        }
+        out[f] = wall ? 0.0 : v;
+    }
+    return FSLBM_OK;
+}
+
+int fslbm_stats_config(fslbm_t c, int xplane, int d2)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (xplane < 0 || xplane >= c->g.nx) return fail(c, FSLBM_EINVAL, "x plane out of range");
+    if (d2 < 0 || d2 > c->g.nz) return fail(c, FSLBM_EINVAL, "d2 out of range");
+    const int n = c->g.ny * (c->g.nz + 2 * d2);
+    if (c->stats && n != c->stats_n) {
+        (void)hipFree(c->stats);
+        c->dev_bytes -= sizeof(double) * 11 * (size_t)c->stats_n;
+        c->stats = nullptr;
+    }
+    if (!c->stats) {
+        if (hipMalloc((void **)&c->stats, sizeof(double) * 11 * (size_t)n) != hipSuccess)
+            return fail(c, FSLBM_ENOMEM, "statistics allocation failed");
+        c->dev_bytes += sizeof(double) * 11 * (size_t)n;
+    }
+    c->stats_n = n;
+    c->stats_x = xplane;
+    c->stats_d2 = d2;
+    c->cnt = c->cnt2 = 0;
+    HIPCHK(c, launch_zero(c->stats, 11ll * n, c->stream));
+    return FSLBM_OK;
+}
+
+int fslbm_stats_points(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    return c->stats_n;
+}
+
+int fslbm_stats_accumulate(fslbm_t c, int delta)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!c->stats) return fail(c, FSLBM_ESTATE, "call fslbm_stats_config first");
+    if (delta < 0 || delta > 2) return fail(c, FSLBM_EINVAL, "delta must be 0, 1 or 2");
+    const long long n = c->stats_n;
+    if (delta == 0) {  // FSLBM.cu:683-686
+        c->cnt = c->cnt2 = 0;
+        HIPCHK(c, launch_zero(c->stats, 11 * n, c->stream));
+    } else if (delta == 1) {  // FSLBM.cu:688-690
+        c->cnt2 = 0;
+        HIPCHK(c, launch_zero(c->stats + 4 * n, 7 * n, c->stream));
+    }
+    c->cnt += 1;
+    c->cnt2 += 1;
+    HIPCHK(c, launch_stats(c->g, c->precision, c->state[c->cur], c->mask, c->q,
+                           c->prm.pin, c->stats_x, c->stats_d2, c->stats,
+                           c->cnt, c->stream));
+    return FSLBM_OK;
+}
+
+int fslbm_stats_read(fslbm_t c, double *acc, int *cnt, int *cnt2)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!c->stats) return fail(c, FSLBM_ESTATE, "call fslbm_stats_config first");
+    if (acc) {
+        HIPCHK(c, hipMemcpyAsync(acc, c->stats, sizeof(double) * 11 * (size_t)c->stats_n,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (cnt) *cnt = c->cnt;
+    if (cnt2) *cnt2 = c->cnt2;
+    return FSLBM_OK;
+}
+
+int fslbm_stats_load(fslbm_t c, const double *acc, int cnt, int cnt2)
+{
+    if (!c || !acc) return FSLBM_EINVAL;
+    if (!c->stats) return fail(c, FSLBM_ESTATE, "call fslbm_stats_config first");
+    if (cnt < 0 || cnt2 < 0) return fail(c, FSLBM_EINVAL, "negative counts");
+    HIPCHK(c, hipMemcpyAsync(c->stats, acc, sizeof(double) * 11 * (size_t)c->stats_n,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cnt = cnt;
+    c->cnt2 = cnt2;
+    return FSLBM_OK;
+}
+
+int fslbm_moments_reset(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    const size_t n = 11 * (size_t)c->g.nx * c->g.ny;
+    if (!c->mom) {
+        if (hipMalloc((void **)&c->mom, n * sizeof(double)) != hipSuccess)
+            return fail(c, FSLBM_ENOMEM, "moment buffer allocation failed");
+        c->dev_bytes += n * sizeof(double);
+    }
+    HIPCHK(c, launch_zero(c->mom, (long long)n, c->stream));
+    c->mom_samples = 0;
+    return FSLBM_OK;
+}
+
+int fslbm_moments_accumulate(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!c->mom) {
+        int rc = fslbm_moments_reset(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, launch_moments(c->g, c->precision, c->state[c->cur], c->mask, c->q,
+                             c->mom, c->stream));
+    c->mom_samples += 1;
+    return FSLBM_OK;
+}
+
+int fslbm_moments_read(fslbm_t c, double *out, long *samples)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!c->mom) return fail(c, FSLBM_ESTATE, "no moments accumulated");
+    if (out) {
+        HIPCHK(c, hipMemcpyAsync(out, c->mom, sizeof(double) * 11 * (size_t)c->g.nx * c->g.ny,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (samples) *samples = c->mom_samples;
+    return FSLBM_OK;
+}
+
+int fslbm_sync(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSLBM_OK;
+}
+
+int fslbm_time_steps(fslbm_t c, int nsteps, double *ms)
+{
+    if (!c || !ms) return FSLBM_EINVAL;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int rc = fslbm_step(c, nsteps);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float f = 0;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return FSLBM_OK;
+}
+
+int fslbm_device_bytes(fslbm_t c, size_t *bytes)
+{
+    if (!c || !bytes) return FSLBM_EINVAL;
+    *bytes = c->dev_bytes;
+    return FSLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the host API and the HIP kernels of
+// libfslbm.so.  Not part of the public C ABI (include/fslbm.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fslbm.h"
+
+namespace fslbm {
+
+// Geometry of one folded domain.
+struct Geo {
+    int nx, ny, nz;
+    long long N;    // nx*ny*nz
+    int periodic;   // 1: fully periodic box, no mask
+};
+
+// Dequantisation of the 16-bit shifted storage modes:
+//   p = pbase + pslope*z + sp*h_p,   u = su*h_u.
+// Powers of two for sp/su keep the scaling exact.
+struct Quant {
+    float sp, inv_sp, su, inv_su, pbase, pslope;
+};
+
+// Per-launch scalars of one step (FSLBM.cu:231,269,599,623-624), in the
+// arithmetic type R of the mode.
+template <typename R>
+struct StepK {
+    R w0, w1, w2;     // weights of the cx=0 plane: centre, face, edge
+    R quarter;        // factor for cx = +-1
+    R dtddy, m, c, cc;
+    R c0, c1, c2;
+    R pin, gauge;
+    int d1;
+};
+
+// Host-side description of a step launch.
+struct StepArgs {
+    double tblend, pin, nu, blend, gauge;
+    int d1;
+    Quant q;
+};
+
+// Build the per-mode constants: exact rationals for FSLBM_F64, the
+// reference's float literals for the float-arithmetic modes.
+StepK<double> make_consts_f64(const StepArgs &a);
+StepK<float> make_consts_f32(const StepArgs &a);
+
+size_t storage_bytes(int precision);  // bytes per stored value
+
+// Kernel launchers (fslbm_kernels.hip, fslbm_tiled.hip).  `in`/`out` are
+// the base of four SoA planes of N stored values each.
+hipError_t launch_step_plain(const Geo &g, int precision, const void *in,
+                             void *out, const uint8_t *mask,
+                             const StepArgs &a, hipStream_t s);
+// Returns hipErrorNotSupported when the tiled kernel cannot take the case.
+hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
+                             void *out, const uint8_t *mask,
+                             const StepArgs &a, hipStream_t s);
+bool tiled_supported(const Geo &g, int precision, int d1);
+
+// Whole-field reductions (8 doubles, see fslbm_reduce) into dst (device).
+hipError_t launch_reduce(const Geo &g, int precision, const void *state,
+                         const uint8_t *mask, const Quant &q, double *partials,
+                         int npart, double *dst, hipStream_t s);
+// Plane-0 monitoring sums: dst[0] = count, dst[1] = sum p, dst[2] = sum uz.
+hipError_t launch_monitor(const Geo &g, int precision, const void *state,
+                          const uint8_t *mask, const Quant &q,
+                          double *partials, int npart, double *dst,
+                          hipStream_t s);
+// Padded-layout plane extraction (fslbm_get_plane) into dst (device).
+hipError_t launch_plane(const Geo &g, int precision, const void *state,
+                        const uint8_t *mask, const Quant &q, double pin,
+                        int axis, int idx, int d2, int field, double *dst,
+                        hipStream_t s);
+// Reference statistics sample (FSLBM.cu:715-748) on the padded YZ plane.
+hipError_t launch_stats(const Geo &g, int precision, const void *state,
+                        const uint8_t *mask, const Quant &q, double pin,
+                        int xplane, int d2, double *acc, int cnt,
+                        hipStream_t s);
+// Moment sums per cross-section cell.
+hipError_t launch_moments(const Geo &g, int precision, const void *state,
+                          const uint8_t *mask, const Quant &q, double *acc,
+                          hipStream_t s);
+// Fill n doubles with zero (kernel, usable under graph capture).
+hipError_t launch_zero(double *dst, long long n, hipStream_t s);
+
+}  // namespace fslbm

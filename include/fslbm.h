@@ -1,0 +1,221 @@
+/*
+ * fslbm.h -- C ABI of the MI355X (gfx950) FSLBM time-step library
+ * (flowcontrast_amd/libfslbm.so).
+ *
+ * The reference (alparmat/FlowContrast, PipeDemoStandalone/FSLBM.cu) has no
+ * library interface: its hot path is the per-step launch sequence
+ *     38 x FSLBMcalc -> 310 x shiftback -> bcond          FSLBM.cu:622-628
+ * over state reached through module-scope texture/surface references
+ * (FSLBM.cu:39-40,537-543), driven by main() and a file protocol
+ * (pipe.txt, param.txt in; qp.txt and the averaged planes out).
+ * This header is the drop-in boundary for that path: one opaque context
+ * owns the device state; plain pointers and sizes cross the boundary.
+ *
+ * Layout handed across the boundary ("folded"): nx*ny*nz cells, x fastest,
+ * then y, then z.  The reference's d2 periodic padding planes
+ * (pipe.m:49-58) are not stored: z is periodic and a gather that wraps
+ * below z=0 adds +pin to the neighbour pressure, one that wraps above
+ * z=nz-1 adds -pin (the copies bcond makes, FSLBM.cu:294-301).
+ * Mask codes: 0 = fluid (updated), 125 = wall (state held at 0).  A NULL
+ * mask selects a fully periodic box (x and y wrap too; no walls).
+ * Stored velocity keeps the reference sign (stored u = -physical u).
+ *
+ * Conventions: every int-returning call returns 0 on success or a
+ * negative FSLBM_E* code; fslbm_last_error() holds the message.  The
+ * caller owns all host buffers; the library keeps no host pointer.  One
+ * context per host thread at a time; many contexts per device.  Calls on a
+ * context are asynchronous on its own HIP stream unless they return host
+ * data (get_state, monitor, reductions, stats reads), which synchronise.
+ */
+#ifndef FSLBM_H
+#define FSLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSLBM_ABI_VERSION 1
+
+/* error codes */
+#define FSLBM_OK 0
+#define FSLBM_EINVAL (-1)   /* bad argument / geometry / parameters */
+#define FSLBM_ENOMEM (-2)   /* device or host allocation failed */
+#define FSLBM_EHIP (-3)     /* HIP runtime error */
+#define FSLBM_ENODEV (-4)   /* no usable gfx950 device */
+#define FSLBM_ESTATE (-5)   /* call not valid in the context's state */
+
+/* storage / arithmetic modes */
+#define FSLBM_F64 0   /* double storage, double arithmetic, exact rationals */
+#define FSLBM_F32 1   /* float storage and arithmetic, reference literals */
+#define FSLBM_F16S 2  /* IEEE half storage of shifted fields, float math */
+#define FSLBM_BF16S 3 /* bfloat16 storage of shifted fields, float math */
+
+/* kernel variants (for testing; AUTO picks the fastest valid one) */
+#define FSLBM_KERNEL_AUTO 0
+#define FSLBM_KERNEL_PLAIN 1 /* one thread per cell, direct 27-point gather */
+#define FSLBM_KERNEL_TILED 2 /* z-marching LDS tile, separable predictor */
+
+/* Run parameters: the six values of param.txt (pipe.m:67-69, read at
+ * FSLBM.cu:499-510 and reloaded at :669-677). */
+typedef struct {
+    double tblend; /* velocity blend factor (timestep = dx * tblend) */
+    double pin;    /* pressure jump across the z period */
+    double nu;     /* lattice viscosity; corrector uses nu - 1/6 */
+    double blend;  /* CCS order 2 <-> 4 blend b in [0,1] */
+    int d1;        /* coarse corrective stencil length (>= 1) */
+    int delta;     /* statistics reset switch 0/1/2, FSLBM.cu:683-690 */
+} fslbm_params;
+
+typedef struct fslbm_ctx *fslbm_t;
+
+/* Library / device information. */
+int fslbm_abi_version(void);
+const char *fslbm_error_string(int code);
+
+/* Create a context and its device state (two state copies, mask).
+ * Replaces the allocation + texture binding of FSLBM.cu:519-543,593.
+ * mask: nx*ny*nz codes (0 fluid / 125 wall), or NULL for a fully
+ * periodic box.  For a masked domain no fluid cell may lie on the x or y
+ * faces of the box (the reference's wall frame, pipe.m:56-58).
+ * The initial state is all zero. */
+int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
+                 int nz, int precision, fslbm_t *out);
+void fslbm_destroy(fslbm_t ctx);
+const char *fslbm_last_error(fslbm_t ctx);
+
+/* Replace the run parameters (param.txt hot reload, FSLBM.cu:669-677). */
+int fslbm_set_params(fslbm_t ctx, const fslbm_params *prm);
+int fslbm_get_params(fslbm_t ctx, fslbm_params *prm);
+
+/* Choose the kernel variant (FSLBM_KERNEL_*). */
+int fslbm_set_kernel(fslbm_t ctx, int variant);
+int fslbm_get_kernel(fslbm_t ctx);
+
+/* Upload / download the whole state as four SoA arrays of nx*ny*nz values
+ * (p, ux, uy, uz).  The uploaded state is taken as the state after a
+ * bcond: wall cells are forced to 0 (FSLBM.cu:290).  Replaces the restart
+ * load (initphys1, FSLBM.cu:386-419) and the backup dump (:839-856). */
+int fslbm_set_state(fslbm_t ctx, const double *p, const double *ux,
+                    const double *uy, const double *uz);
+int fslbm_get_state(fslbm_t ctx, double *p, double *ux, double *uy,
+                    double *uz);
+int fslbm_set_state_f32(fslbm_t ctx, const float *p, const float *ux,
+                        const float *uy, const float *uz);
+int fslbm_get_state_f32(fslbm_t ctx, float *p, float *ux, float *uy,
+                        float *uz);
+
+/* Advance nsteps time steps: predictor + CCS corrector + blend + periodic
+ * and wall boundaries (FSLBM.cu:622-628).  A pending gauge (set by
+ * fslbm_monitor or fslbm_set_gauge) is subtracted from the new pressure of
+ * the first step only, as bcond's p2 (FSLBM.cu:628). Asynchronous. */
+int fslbm_step(fslbm_t ctx, int nsteps);
+
+/* Set the gauge p2 for the next step (FSLBM.cu:628). */
+int fslbm_set_gauge(fslbm_t ctx, double p2);
+
+/* Monitoring, FSLBM.cu:630-647: out[0..3] = q1, q2, p1, p2, the mean axial
+ * velocity and pressure over the inlet plane (padded z=d2, folded z=0) and
+ * the outlet plane (padded z=lz-d2, the periodic image of folded z=0 with
+ * p - pin), each divided by the fluid cell count of that cross-section.
+ * If set_gauge != 0, p2 becomes the gauge of the next step (the
+ * reference always does this).  Sums are FP64 on the device. */
+int fslbm_monitor(fslbm_t ctx, int set_gauge, double out[4]);
+
+/* Whole-field FP64 reductions over non-wall cells:
+ * out[0] = count, out[1] = sum p, out[2..4] = sum ux, uy, uz,
+ * out[5] = sum |u|^2, out[6] = sum p^2, out[7] = number of non-finite
+ * values (NaN/Inf) in the state. */
+int fslbm_reduce(fslbm_t ctx, double out[8]);
+
+/* Single value at (x,y,z) (the qp.txt probe, FSLBM.cu:653-665):
+ * out[0..3] = p, ux, uy, uz. */
+int fslbm_probe(fslbm_t ctx, int x, int y, int z, double out[4]);
+
+/* Extract one plane in the reference's padded layout.
+ * axis 0: YZ plane at x = idx, out has ny*(nz+2*d2) values, index
+ *         y + ny*zp (getYZplane, FSLBM.cu:315-324);
+ * axis 2: XY plane at padded z = idx, out has nx*ny values, index x + nx*y
+ *         (getXYplane, FSLBM.cu:327-336).
+ * field 0..3 = p, ux, uy, uz.  Padding planes are rebuilt from the
+ * periodic images with +pin (inlet) / -pin (outlet), walls are 0. */
+int fslbm_get_plane(fslbm_t ctx, int axis, int idx, int d2, int field,
+                    double *out);
+
+/* Reference plane statistics, FSLBM.cu:683-764, accumulated on the device
+ * in FP64 over the YZ plane x = xplane in the padded layout with d2
+ * padding planes (n = ny*(nz+2*d2) points).  Each accumulate call is one
+ * sample: optional reset per delta, cnt += 1, cnt2 += 1, running sums,
+ * |dev| and dev products against the running mean including the sample.
+ * read: acc gets 11*n values (sum p, ux, uy, uz; sum |dev| p, ux, uy, uz;
+ * sum dev_x dev_y, dev_x dev_z, dev_z dev_y) and the counts; divide by cnt
+ * (first four) or cnt2 (the rest) as the reference does at write time.
+ * load: restore accumulators (restart, FSLBM.cu:429-468). */
+int fslbm_stats_config(fslbm_t ctx, int xplane, int d2);
+int fslbm_stats_accumulate(fslbm_t ctx, int delta);
+int fslbm_stats_read(fslbm_t ctx, double *acc, int *cnt, int *cnt2);
+int fslbm_stats_load(fslbm_t ctx, const double *acc, int cnt, int cnt2);
+int fslbm_stats_points(fslbm_t ctx);
+
+/* True moments for turbulence statistics (not in the reference, which
+ * keeps mean |dev|): per cross-section cell (x,y), FP64 sums over z and
+ * over samples of p, ux, uy, uz, p^2, ux^2, uy^2, uz^2, ux*uy, ux*uz,
+ * uy*uz.  read: out gets 11*nx*ny values, *samples the number of
+ * accumulate calls (each call adds nz values per (x,y)). */
+int fslbm_moments_reset(fslbm_t ctx);
+int fslbm_moments_accumulate(fslbm_t ctx);
+int fslbm_moments_read(fslbm_t ctx, double *out, long *samples);
+
+/* Wait for all work queued on the context. */
+int fslbm_sync(fslbm_t ctx);
+
+/* Time nsteps steps with HIP events recorded on the context's stream
+ * around the whole sequence: *ms gets the elapsed milliseconds. */
+int fslbm_time_steps(fslbm_t ctx, int nsteps, double *ms);
+
+/* Bytes of device memory the context holds. */
+int fslbm_device_bytes(fslbm_t ctx, size_t *bytes);
+
+/* Host-side input builders (no device work).
+ *
+ * pipe.m geometry (pipe.m:38-58): inner box of side `diameter`, circle
+ * ((i-o)^2 + (j-o)^2 <= r^2 with 1-based i,j, o = r = diameter/2), a wall
+ * frame of `frame` cells, length + 2*frame planes; codes 0 fluid,
+ * 1 padding (first and last `frame` planes), 125 wall.  mask has
+ * (diameter+2*frame)^2*(length+2*frame) bytes, x fastest.
+ * Returns the number of padding cells (code 1). */
+long fslbm_pipe_m_mask(int diameter, int length, int frame, uint8_t *mask);
+
+/* Circular pipe for the folded layout: nx*ny*nz codes, fluid where
+ * (x - cx)^2 + (y - cy)^2 <= R^2 with cx = (nx-1)/2, cy = (ny-1)/2
+ * (0-based), wall elsewhere.  Returns the fluid cells per cross-section. */
+long fslbm_circle_mask(int nx, int ny, int nz, double radius, uint8_t *mask);
+
+/* Seeded initial states (SplitMix64 keyed by seed, cell and component;
+ * identical on every host).
+ * Poiseuille: p = pin * (nz - z) / nz (the initphys ramp, FSLBM.cu:381,
+ * folded), stored uz = -2*ubar*(1 - r^2/R^2) on fluid cells, plus uniform
+ * noise in [-noise, noise] * 2*ubar on each velocity component; walls 0.
+ * Taylor-Green (fully periodic, 2*pi box): u = U0 sin x cos y cos z,
+ * v = -U0 cos x sin y cos z, w = 0 (stored with the reference sign, i.e.
+ * negated), p = rho0 + 3*(U0^2/16)(cos 2x + cos 2y)(cos 2z + 2). */
+int fslbm_init_poiseuille(int nx, int ny, int nz, const uint8_t *mask,
+                          double radius, double ubar, double pin, double noise,
+                          uint64_t seed, double *p, double *ux, double *uy,
+                          double *uz);
+int fslbm_init_taylor_green(int nx, int ny, int nz, double u0, double rho0,
+                            double *p, double *ux, double *uy, double *uz);
+
+/* Fold a reference (padded) pipe.txt mask of lx*ly*lz codes with d2
+ * padding planes per end into nx=lx, ny=ly, nz=lz-2*d2 folded codes
+ * (0/125).  Fails if the padding planes are not periodic images of the
+ * first/last fluid planes. */
+int fslbm_fold_mask(int lx, int ly, int lz, int d2, const uint8_t *padded,
+                    uint8_t *folded);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSLBM_H */

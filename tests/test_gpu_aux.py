@@ -1,0 +1,220 @@
+"""GPU checks of the on-device reductions, padded plane views, reference
+statistics and moments against the oracle / numpy on the same state, and
+size-independent properties of the step at the full 256x256x1024 and
+256^3 sizes."""
+import math
+
+import numpy as np
+import pytest
+
+from flowcontrast_amd import KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver, inputs
+from flowcontrast_amd._lib import FslbmError
+
+pytestmark = pytest.mark.gpu
+
+
+def small_pipe(precision="f64"):
+    nx, ny, nz, R = 24, 22, 18, 9.0
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    pin = 3e-3
+    st = inputs.poiseuille_state(mask, R, 0.05, pin, 0.05, 3)
+    prm = RunParams(tblend=0.9, pin=pin, nu=0.02, d1=2, delta=1, blend=0.5)
+    s = Solver(mask.shape, prm, mask=mask, precision=precision)
+    s.set_state(*st)
+    return s, mask, st, prm
+
+
+def padded_view(st, mask, d2, pin):
+    nz = st[0].shape[0]
+    lz = nz + 2 * d2
+    zs = (np.arange(lz) - d2) % nz
+    jump = np.where(np.arange(lz) < d2, pin, np.where(np.arange(lz) >= lz - d2, -pin, 0.0))
+    out = [np.asarray(s, dtype=np.float64)[zs].copy() for s in st]
+    out[0] = out[0] + jump[:, None, None]
+    wall = mask[zs] != 0
+    for f in range(4):
+        out[f][wall] = 0.0
+    return out
+
+
+def test_reduce_matches_numpy():
+    s, mask, st, prm = small_pipe()
+    with s:
+        s.step(3)
+        got = s.get_state()
+        r = s.reduce()
+    fl = mask == 0
+    assert r[0] == fl.sum()
+    assert abs(r[1] - got[0][fl].sum()) < 1e-12
+    for f in (1, 2, 3):
+        assert abs(r[1 + f] - got[f][fl].sum()) < 1e-13
+    assert abs(r[5] - sum((got[f][fl] ** 2).sum() for f in (1, 2, 3))) < 1e-13
+    assert abs(r[6] - (got[0][fl] ** 2).sum()) < 1e-12
+    assert r[7] == 0
+
+
+def test_nan_is_detected():
+    s, mask, st, prm = small_pipe()
+    with s:
+        bad = [a.copy() for a in st]
+        i = np.argwhere(mask == 0)[5]
+        bad[2][tuple(i)] = np.nan
+        s.set_state(*bad)
+        assert s.reduce()[7] == 1
+        s.step(1)
+        assert s.reduce()[7] > 1  # spreads to the neighbours
+
+
+def test_plane_views_and_probe(orc):
+    s, mask, st, prm = small_pipe()
+    with s:
+        s.step(2)
+        got = s.get_state()
+        d2 = 4
+        pv = padded_view(got, mask, d2, prm.pin)
+        nz, ny, nx = mask.shape
+        for f in range(4):
+            yz = s.plane(0, nx // 2, d2, f)
+            assert np.array_equal(yz, pv[f][:, :, nx // 2].reshape(-1))
+            xy = s.plane(2, d2 + nz // 2, d2, f)
+            assert np.array_equal(xy, pv[f][d2 + nz // 2].reshape(-1))
+        pr = s.probe(nx // 2, ny // 2, nz // 2)
+        assert np.array_equal(pr, [got[f][nz // 2, ny // 2, nx // 2] for f in range(4)])
+
+
+def test_reference_stats_match_oracle(orc):
+    s, mask, st, prm = small_pipe()
+    d2 = 3
+    nz, ny, nx = mask.shape
+    xplane = nx // 2
+    o = orc.Stats(ny * (nz + 2 * d2))
+    with s:
+        s.stats_config(xplane, d2)
+        for k, delta in enumerate([2, 2, 1, 2, 2, 0, 2]):
+            s.step(5)
+            s.stats_accumulate(delta)
+            got = s.get_state()
+            pv = padded_view(got, mask, d2, prm.pin)
+            o.accumulate([pv[f][:, :, xplane].reshape(-1) for f in range(4)], delta)
+        acc, cnt, cnt2 = s.stats_read()
+    ref, rc, rc2 = o.read()
+    assert (cnt, cnt2) == (rc, rc2)
+    assert np.abs(acc - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+
+
+def test_stats_load_roundtrip():
+    s, mask, st, prm = small_pipe()
+    with s:
+        s.stats_config(5, 2)
+        s.stats_accumulate(2)
+        s.step(2)
+        s.stats_accumulate(2)
+        acc, cnt, cnt2 = s.stats_read()
+        s.stats_config(5, 2)
+        s.stats_load(acc, cnt, cnt2)
+        acc2, c, c2 = s.stats_read()
+    assert np.array_equal(acc, acc2) and (c, c2) == (cnt, cnt2)
+
+
+def test_moments_match_numpy():
+    s, mask, st, prm = small_pipe()
+    with s:
+        s.moments_reset()
+        tot = np.zeros((11,) + mask.shape[1:])
+        for _ in range(3):
+            s.step(4)
+            s.moments_accumulate()
+            g = s.get_state()
+            p, u, v, w = [np.where(mask == 0, a, 0.0) for a in g]
+            for k, q in enumerate([p, u, v, w, p * p, u * u, v * v, w * w, u * v, u * w, v * w]):
+                tot[k] += q.sum(axis=0)
+        mom, n = s.moments_read()
+    assert n == 3
+    assert np.abs(mom - tot).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["f16s", "bf16s"])
+def test_shifted_storage_roundtrip(precision):
+    s, mask, st, prm = small_pipe(precision)
+    with s:
+        got = s.get_state()
+    umax = max(np.abs(st[f]).max() for f in (1, 2, 3))
+    rel = 2 ** -11 if precision == "f16s" else 2 ** -8
+    for f in (1, 2, 3):
+        assert np.abs(got[f] - st[f]).max() <= 1.01 * rel * 2 * umax
+    assert np.abs(got[0] - st[0]).max() <= 1e-5
+    assert (got[0][mask != 0] == 0).all()
+
+
+# ----------------------------------------------------------------------
+# size-independent properties at the benchmark sizes
+# ----------------------------------------------------------------------
+def _variants():
+    return [KERNEL_PLAIN, KERNEL_TILED]
+
+
+@pytest.mark.parametrize("variant", _variants())
+def test_full_size_shear_wave_is_exact_eigenmode(variant):
+    """256^3 periodic box, FP32: u = A sin(kz) x-hat decays by exactly
+    lambda(k) per step (the update is linear on this subspace)."""
+    n = 256
+    k = 2 * math.pi * 4 / n
+    tb, nu, b, d1, A = 1.0, 0.01, 1.0, 2, 0.05
+    prm = RunParams(tblend=tb, pin=0.0, nu=nu, d1=d1, delta=1, blend=b)
+    z = np.arange(n, dtype=np.float64)
+    prof = (A * np.sin(k * z))[:, None, None] * np.ones((1, n, n))
+    p = np.full((n, n, n), 0.25, dtype=np.float32)
+    zero = np.zeros((n, n, n), dtype=np.float32)
+    try:
+        s = Solver((n, n, n), prm, mask=None, precision="f32")
+        s.set_kernel(variant)
+    except FslbmError:
+        pytest.skip("variant unsupported")
+    with s:
+        s.set_state(p, prof.astype(np.float32), zero, zero)
+        s.step(10)
+        got = s.get_state(np.float32)
+    c1, c2 = 1 + b / 3, -b / 12
+    lam = (1 - tb) + tb * ((2 + math.cos(k)) / 3 + (nu - 1 / 6) / d1 ** 2 *
+                           (-2 * c1 * (1 - math.cos(k * d1)) - 2 * c2 * (1 - math.cos(2 * k * d1))))
+    exp = lam ** 10 * prof
+    assert np.abs(got[1] - exp).max() < 1e-6 * A
+    assert np.abs(got[0] - 0.25).max() < 1e-6
+    assert np.abs(got[2]).max() < 1e-7 and np.abs(got[3]).max() < 1e-7
+
+
+@pytest.mark.parametrize("variant", _variants())
+def test_full_size_pipe_conserves_subspace(variant):
+    """256x256x1024 pipe, FP32: a z-invariant axial flow with the linear
+    pressure ramp stays axial (ux = uy = 0, p unchanged); the rest state is
+    fixed (size-independent properties of the update)."""
+    nx = ny = 256
+    nz = 1024
+    R = 120.0
+    mask, nf = inputs.circle_mask(nx, ny, nz, R)
+    pin = 1e-3
+    G = pin / nz
+    fl = mask == 0
+    zc = np.arange(nz, dtype=np.float32)[:, None, None]
+    p = np.where(fl, np.float32(0.01) - np.float32(G) * zc, 0).astype(np.float32)
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    r2 = ((xx - 127.5) ** 2 + (yy - 127.5) ** 2) / R ** 2
+    w2 = (-0.05 * np.clip(1 - r2, 0, None)).astype(np.float32)
+    w = np.where(fl, w2[None], 0).astype(np.float32)
+    zero = np.zeros_like(p)
+    prm = RunParams(tblend=1.0, pin=pin, nu=0.01, d1=2, delta=1, blend=0.0)
+    try:
+        s = Solver(mask.shape, prm, mask=mask, precision="f32")
+        s.set_kernel(variant)
+    except FslbmError:
+        pytest.skip("variant unsupported")
+    with s:
+        s.set_state(p, zero, zero, w)
+        s.step(4)
+        got = s.get_state(np.float32)
+        r = s.reduce()
+    assert r[7] == 0
+    assert np.abs(got[1]).max() < 1e-9 and np.abs(got[2]).max() < 1e-9
+    assert np.abs(got[0] - p).max() < 2e-8
+    # z-invariance of the axial velocity is kept exactly up to roundoff
+    assert np.abs(got[3] - got[3][:1]).max() < 1e-8

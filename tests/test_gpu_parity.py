@@ -1,0 +1,172 @@
+"""GPU parity of the HIP step against the CPU oracle, through the C ABI.
+
+Tolerances (max |gpu - oracle| / max |oracle| per field unless noted):
+  FP64 GPU vs FP64 oracle           <= 1e-12 after 100 and 1000 steps
+  FP32 GPU vs FP64 oracle           relative L2 <= 1e-5 (velocity, p)
+  FP32 GPU vs FP32 oracle           relative L2 <= 1e-5
+Every kernel variant that supports the case is checked.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flowcontrast_amd import KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver, inputs
+from flowcontrast_amd._lib import FslbmError
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [KERNEL_PLAIN, KERNEL_TILED]
+
+
+def max_rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def c1_case(blend=0.0):
+    """32x32x64 pipe, R=14, FP64, Re_b ~ 200 (nu = 0.01, ubar = 0.0714)."""
+    nx = ny = 32
+    nz = 64
+    R, nu = 14.0, 0.01
+    ubar = 200 * nu / (2 * R)
+    pin = inputs.laminar_pin(nu, nz, ubar, R)
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    st = inputs.poiseuille_state(mask, R, ubar, pin, 0.01, 1)
+    prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=blend)
+    return mask, st, prm
+
+
+def orc_params(orc, prm):
+    return orc.params(prm.tblend, prm.pin, prm.nu, prm.blend, prm.d1)
+
+
+def make_solver(shape, prm, mask, precision, variant):
+    s = Solver(shape, prm, mask=mask, precision=precision)
+    try:
+        s.set_kernel(variant)
+    except FslbmError:
+        s.close()
+        pytest.skip("kernel variant does not support this case")
+    return s
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("blend", [0.0, 1.0])
+def test_c1_pipe_f64_100_steps(orc, variant, blend):
+    mask, st, prm = c1_case(blend)
+    ref = orc.step_folded(st, orc_params(orc, prm), mask=mask, nsteps=100)
+    with make_solver(mask.shape, prm, mask, "f64", variant) as s:
+        s.set_state(*st)
+        s.step(100)
+        got = s.get_state()
+    for f in range(4):
+        assert max_rel(got[f], ref[f]) <= 1e-12, f
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_c1_pipe_f64_1000_steps_with_gauge(orc, variant):
+    """The reference cadence: monitoring every 50 steps sets the gauge p2
+    subtracted at the next step (FSLBM.cu:628-647)."""
+    mask, st, prm = c1_case(0.0)
+    op = orc_params(orc, prm)
+    ref = [np.array(a) for a in st]
+    with make_solver(mask.shape, prm, mask, "f64", variant) as s:
+        s.set_state(*st)
+        gauge = 0.0
+        at100 = None
+        for block in range(20):
+            ref = orc.step_folded(ref, op, mask=mask, gauge=gauge, nsteps=50)
+            s.step(50)
+            fluid0 = mask[0] == 0
+            p1 = ref[0][0][fluid0].sum() / fluid0.sum()
+            gauge = p1 - prm.pin
+            mon = s.monitor(set_gauge=True)
+            assert abs(mon[2] - p1) <= 1e-12 * max(1.0, abs(p1))
+            assert abs(mon[3] - gauge) <= 1e-12 * max(1.0, abs(p1))
+            assert abs(mon[0] - ref[3][0][fluid0].sum() / fluid0.sum()) < 1e-14
+            if block == 1:
+                at100 = s.get_state()
+                for f in range(4):
+                    assert max_rel(at100[f], ref[f]) <= 1e-12
+        got = s.get_state()
+    for f in range(4):
+        assert max_rel(got[f], ref[f]) <= 1e-12, f
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_c1_pipe_f32_vs_f64_oracle(orc, variant):
+    mask, st, prm = c1_case(0.0)
+    ref = orc.step_folded(st, orc_params(orc, prm), mask=mask, nsteps=100)
+    ref32 = orc.step_folded(st, orc_params(orc, prm), mask=mask, nsteps=100,
+                            dtype=np.float32)
+    with make_solver(mask.shape, prm, mask, "f32", variant) as s:
+        s.set_state(*[a.astype(np.float32) for a in st])
+        s.step(100)
+        got = s.get_state()
+    for f in range(4):
+        assert rel_l2(got[f], ref[f]) <= 1e-5, f
+        assert rel_l2(got[f], ref32[f]) <= 1e-5, f
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_taylor_green_periodic(orc, variant, precision, tol):
+    n = 32
+    nu = inputs.tgv_viscosity(n, 0.05, 1600.0)
+    st = inputs.taylor_green_state(n, 0.05, 1.0)
+    prm = RunParams(tblend=1.0, pin=0.0, nu=nu, d1=2, delta=1, blend=0.0)
+    ref = orc.step_folded(st, orc_params(orc, prm), nsteps=100)
+    with make_solver((n, n, n), prm, None, precision, variant) as s:
+        s.set_state(*st)
+        s.step(100)
+        got = s.get_state()
+    for f in range(4):
+        err = max_rel(got[f], ref[f]) if precision == "f64" else rel_l2(got[f], ref[f])
+        if f == 3:  # w stays ~0 (roundoff only): compare absolutely
+            bound = 1e-12 if precision == "f64" else 1e-6
+            assert np.abs(got[f]).max() < bound
+            continue
+        assert err <= tol, (f, err)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_odd_shapes_and_d1(orc, variant):
+    """Ragged sizes (not multiples of the tile), d1 = 1 and 3, blend."""
+    for (nx, ny, nz, R, d1, blend) in [(37, 29, 23, 11.0, 1, 0.5),
+                                       (70, 66, 17, 30.0, 3, 1.0),
+                                       (9, 9, 8, 3.0, 2, 0.0)]:
+        mask, _ = inputs.circle_mask(nx, ny, nz, R)
+        pin = 2e-3
+        st = inputs.poiseuille_state(mask, R, 0.03, pin, 0.05, 5)
+        prm = RunParams(tblend=0.9, pin=pin, nu=0.02, d1=d1, delta=1, blend=blend)
+        ref = orc.step_folded(st, orc_params(orc, prm), mask=mask, gauge=1e-3, nsteps=7)
+        with make_solver(mask.shape, prm, mask, "f64", variant) as s:
+            s.set_state(*st)
+            s.set_gauge(1e-3)
+            s.step(7)
+            got = s.get_state()
+        for f in range(4):
+            assert max_rel(got[f], ref[f]) <= 1e-12, (nx, ny, nz, f)
+
+
+def test_plain_and_tiled_agree_f32():
+    mask, st, prm = c1_case(1.0)
+    outs = []
+    for v in VARIANTS:
+        try:
+            with make_solver(mask.shape, prm, mask, "f32", v) as s:
+                s.set_state(*[a.astype(np.float32) for a in st])
+                s.step(50)
+                outs.append(s.get_state())
+        except pytest.skip.Exception:
+            pass
+    if len(outs) < 2:
+        pytest.skip("only one kernel variant available")
+    for f in range(4):
+        assert rel_l2(outs[0][f], outs[1][f]) <= 1e-6

@@ -1,0 +1,387 @@
+"""Pins of the CPU oracle (test infrastructure) against closed forms.
+
+The reference ships no tests, fixtures or golden vectors, and it cannot
+be built here (CUDA texture/surface references).  Every check below is a
+known answer derived from the update rule of FSLBM.cu:130-303 by hand,
+computed here by a different route than the oracle's 27-point gather:
+
+  fixed points              rest state with walls; uniform flow (periodic)
+  shear-wave eigenmode      exact per-step factor lambda(k) for any axis
+  pressure wave, one step   closed form that fixes the streaming sign
+  discrete Poiseuille       invariant subspace + 2D linear steady state
+  conservation / jump       sum p, sum u exact; d(sum uz) = -tb nx ny pin/3
+  CCS coefficient tables    reference's commented tables (FSLBM.cu:178-231)
+  padded == folded          reference layout vs folded layout
+  statistics                hand-computed sample sequences (FSLBM.cu:715-763)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flowcontrast_amd import inputs
+
+
+def rnd_state(shape, seed, p0=0.0, amp=0.02):
+    r = np.random.default_rng(seed)
+    return [p0 + amp * r.standard_normal(shape) for _ in range(1)] + \
+           [amp * r.standard_normal(shape) for _ in range(3)]
+
+
+# ----------------------------------------------------------------------
+# fixed points
+# ----------------------------------------------------------------------
+def test_rest_state_with_walls_is_fixed(orc):
+    nz, ny, nx = 8, 14, 14
+    mask, _ = inputs.circle_mask(nx, ny, nz, 5.0)
+    p = np.where(mask == 0, 0.37, 0.0)
+    z = np.zeros_like(p)
+    prm = orc.params(tblend=0.8, nu=0.02, blend=0.5, d1=2)
+    out = orc.step_folded([p, z, z, z], prm, mask=mask)
+    assert np.abs(out[0] - p).max() < 1e-15
+    for f in (1, 2, 3):
+        assert np.abs(out[f]).max() < 1e-16
+
+
+def test_uniform_flow_is_fixed_periodic(orc):
+    n = 10
+    U = (0.031, -0.017, 0.024)
+    st = [np.full((n, n, n), 0.5)] + [np.full((n, n, n), u) for u in U]
+    prm = orc.params(tblend=0.6, nu=0.03, blend=1.0, d1=2)
+    out = orc.step_folded(st, prm)
+    for f in range(4):
+        assert np.abs(out[f] - st[f]).max() < 1e-15
+
+
+# ----------------------------------------------------------------------
+# shear-wave eigenmode (exactly linear for any amplitude)
+# ----------------------------------------------------------------------
+def shear_lambda(k, tb, nu, d1, b):
+    c1 = 1 + b / 3
+    c2 = -b / 12
+    smooth = (2 + math.cos(k)) / 3
+    corr = (nu - 1 / 6) / d1 ** 2 * (-2 * c1 * (1 - math.cos(k * d1))
+                                     - 2 * c2 * (1 - math.cos(2 * k * d1)))
+    return (1 - tb) + tb * (smooth + corr)
+
+
+@pytest.mark.parametrize("vel_axis,wave_axis", [(0, 2), (1, 0), (2, 1), (0, 1)])
+@pytest.mark.parametrize("b,d1", [(0.0, 2), (1.0, 2), (0.4, 1), (0.7, 3)])
+def test_shear_wave_eigenvalue(orc, vel_axis, wave_axis, b, d1):
+    n = 16
+    mode = 2
+    k = 2 * math.pi * mode / n
+    tb, nu, A = 0.7, 0.013, 0.05
+    coords = np.indices((n, n, n))  # (z, y, x)
+    wave_coord = coords[2 - wave_axis]
+    prof = A * np.sin(k * wave_coord)
+    st = [np.full((n, n, n), 0.2), np.zeros((n, n, n)), np.zeros((n, n, n)),
+          np.zeros((n, n, n))]
+    st[1 + vel_axis] = prof
+    prm = orc.params(tblend=tb, nu=nu, blend=b, d1=d1)
+    lam = shear_lambda(k, tb, nu, d1, b)
+    out = orc.step_folded(st, prm, nsteps=3)
+    assert np.abs(out[0] - 0.2).max() < 1e-15
+    assert np.abs(out[1 + vel_axis] - lam ** 3 * prof).max() < 1e-15
+    for f in range(3):
+        if f != vel_axis:
+            assert np.abs(out[1 + f]).max() < 1e-16
+
+
+def test_shear_wave_f32_close(orc):
+    n = 16
+    k = 2 * math.pi / n
+    z = np.indices((n, n, n))[0]
+    prof = 0.05 * np.sin(k * z)
+    st = [np.full((n, n, n), 0.2), prof, np.zeros_like(prof), np.zeros_like(prof)]
+    prm = orc.params(tblend=1.0, nu=0.01, blend=0.0, d1=2)
+    out = orc.step_folded(st, prm, dtype=np.float32)
+    lam = shear_lambda(k, 1.0, 0.01, 2, 0.0)
+    assert np.abs(out[1] - lam * prof).max() < 2e-8
+
+
+# ----------------------------------------------------------------------
+# one-step pressure wave: fixes the sign of the streaming
+# ----------------------------------------------------------------------
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_pressure_wave_one_step(orc, axis):
+    n, eps, p0, tb = 16, 1e-3, 0.4, 0.6
+    k = 2 * math.pi * 3 / n
+    c = np.indices((n, n, n))[2 - axis]
+    p = p0 + eps * np.cos(k * c)
+    z = np.zeros_like(p)
+    prm = orc.params(tblend=tb, nu=0.02, blend=0.3, d1=2)
+    out = orc.step_folded([p, z, z, z], prm)
+    assert np.abs(out[0] - (p0 + eps * np.cos(k * c) * (2 + math.cos(k)) / 3)).max() < 1e-15
+    u_exp = -tb * (eps / 3) * math.sin(k) * np.sin(k * c)
+    assert np.abs(out[1 + axis] - u_exp).max() < 1e-17
+    for f in range(3):
+        if f != axis:
+            assert np.abs(out[1 + f]).max() < 1e-16  # roundoff of sum W e p, p ~ 0.4
+
+
+# ----------------------------------------------------------------------
+# discrete Poiseuille: invariant subspace and steady state
+# ----------------------------------------------------------------------
+W1 = {-1: 1 / 6, 0: 2 / 3, 1: 1 / 6}
+
+
+def poiseuille_map(w2d, fluid, G, tb, nu, d1, b):
+    """w <- w + tb[(S_xy - I) w + (nu-1/6)/d1^2 L_d w - (G/3) phi]."""
+    ny, nx = w2d.shape
+    c0, c1, c2 = -6 - 1.5 * b, 1 + b / 3, -b / 12
+    wz = np.where(fluid, w2d, 0.0)
+    S = np.zeros_like(wz)
+    phi = np.zeros_like(wz)
+    for cy in (-1, 0, 1):
+        for cx in (-1, 0, 1):
+            sh = np.roll(np.roll(wz, -cy, 0), -cx, 1)
+            fl = np.roll(np.roll(fluid, -cy, 0), -cx, 1)
+            S += W1[cx] * W1[cy] * sh
+            phi += W1[cx] * W1[cy] * fl
+    def arm(dx, dy):
+        yi = np.clip(np.arange(ny) + dy, 0, ny - 1)
+        xi = np.clip(np.arange(nx) + dx, 0, nx - 1)
+        return wz[np.ix_(yi, xi)]
+    Ld = (c0 + 2 * c1 + 2 * c2) * wz
+    for s, c in ((d1, c1), (2 * d1, c2)):
+        Ld += c * (arm(s, 0) + arm(-s, 0) + arm(0, s) + arm(0, -s))
+    new = wz + tb * ((S - wz) + (nu - 1 / 6) / d1 ** 2 * Ld - (G / 3) * phi)
+    return np.where(fluid, new, 0.0)
+
+
+def test_poiseuille_subspace_one_step(orc):
+    nz, ny, nx = 6, 16, 16
+    R = 6.2
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    fluid = mask[0] == 0
+    pin, tb, nu, b, d1 = 3e-3, 0.9, 0.02, 0.5, 2
+    G = pin / nz
+    r = np.random.default_rng(5)
+    w2d = np.where(fluid, -0.03 + 0.01 * r.standard_normal((ny, nx)), 0.0)
+    zc = np.arange(nz)[:, None, None]
+    p = np.where(mask == 0, 0.11 - G * zc, 0.0)
+    w = np.broadcast_to(w2d, (nz, ny, nx)).copy()
+    zero = np.zeros_like(p)
+    prm = orc.params(tblend=tb, pin=pin, nu=nu, blend=b, d1=d1)
+    out = orc.step_folded([p, zero, zero, w], prm, mask=mask)
+    assert np.abs(out[0] - p).max() < 1e-15
+    assert np.abs(out[1]).max() < 1e-17 and np.abs(out[2]).max() < 1e-17
+    expect = poiseuille_map(w2d, fluid, G, tb, nu, d1, b)
+    assert np.abs(out[3] - expect[None]).max() < 1e-15
+
+
+def test_poiseuille_steady_state(orc):
+    nz, ny, nx = 4, 18, 18
+    R = 7.3
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    fluid = mask[0] == 0
+    pin, tb, nu, b, d1 = 2e-3, 1.0, 0.05, 0.0, 2
+    G = pin / nz
+    # Solve the linear steady state (S - I + g L_d) w = (G/3) phi directly.
+    idx = np.flatnonzero(fluid)
+    n = idx.size
+    M = np.zeros((n, n))
+    base = poiseuille_map(np.zeros((ny, nx)), fluid, G, tb, nu, d1, b).reshape(-1)[idx]
+    for j, cell in enumerate(idx):
+        e = np.zeros(ny * nx)
+        e[cell] = 1.0
+        M[:, j] = poiseuille_map(e.reshape(ny, nx), fluid, G, tb, nu, d1, b).reshape(-1)[idx] \
+            - base - e[idx]
+    wf = np.linalg.solve(M, -base)
+    w2d = np.zeros(ny * nx)
+    w2d[idx] = wf
+    w2d = w2d.reshape(ny, nx)
+    zc = np.arange(nz)[:, None, None]
+    p = np.where(mask == 0, 0.05 - G * zc, 0.0)
+    w = np.broadcast_to(w2d, (nz, ny, nx)).copy()
+    zero = np.zeros_like(p)
+    prm = orc.params(tblend=tb, pin=pin, nu=nu, blend=b, d1=d1)
+    out = orc.step_folded([p, zero, zero, w], prm, mask=mask, nsteps=5)
+    assert np.abs(out[3] - w).max() < 1e-14 * np.abs(w).max()
+    # flow goes down the pressure gradient: stored uz < 0 (stored = -physical)
+    assert w2d[ny // 2, nx // 2] < 0
+
+
+# ----------------------------------------------------------------------
+# conservation and the pressure-jump momentum budget
+# ----------------------------------------------------------------------
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-4)])
+def test_conservation_periodic(orc, dtype, tol):
+    n = 12
+    st = rnd_state((n, n, n), 3, p0=0.3)
+    prm = orc.params(tblend=0.8, nu=0.011, blend=0.6, d1=2)
+    out = orc.step_folded(st, prm, dtype=dtype, nsteps=4)
+    for f in range(4):
+        s0 = np.sum(np.asarray(st[f], dtype=np.float64))
+        s1 = np.sum(np.asarray(out[f], dtype=np.float64))
+        assert abs(s1 - s0) < tol * max(1.0, abs(s0)), f
+
+
+def test_jump_budget(orc):
+    n, nz = 10, 14
+    r = np.random.default_rng(9)
+    st = [0.3 + 0.01 * r.standard_normal((nz, n, n))] + \
+         [0.02 * r.standard_normal((nz, n, n)) for _ in range(3)]
+    pin, tb = 4e-3, 0.7
+    prm = orc.params(tblend=tb, pin=pin, nu=0.02, blend=0.2, d1=2)
+    out = orc.step_folded(st, prm)
+    d_uz = out[3].sum() - st[3].sum()
+    assert abs(d_uz - (-tb * n * n * pin / 3)) < 1e-13
+    assert abs(out[0].sum() - st[0].sum()) < 1e-12
+    assert abs(out[1].sum() - st[1].sum()) < 1e-13
+
+
+# ----------------------------------------------------------------------
+# CCS coefficients: the reference's commented tables
+# ----------------------------------------------------------------------
+def central_second_derivative(N):
+    c = [2 * (-1) ** (j + 1) * math.factorial(N) ** 2 /
+         (j * j * math.factorial(N - j) * math.factorial(N + j)) for j in range(1, N + 1)]
+    return [-6 * sum(c)] + c
+
+
+# (c0, c1, c2, ...) as printed in FSLBM.cu:178-214 (orders 14..2)
+REFERENCE_TABLES = {
+    14: [-9.16453231293, 1.77777777778, -0.31111111111, 0.07542087542,
+         -0.01767676767, 3.48096348e-3, -5.1800052e-4, 5.0742908e-5, -2.4281274e-6],
+    12: [-8.94833333333, 1.71428571429, -0.26785714285, 0.05291005291,
+         -0.00892857142, 0.00103896103, -0.00006012506],
+    10: [-8.78166666666, 1.66666666, -0.23809523809, 0.03968253968,
+         -0.00496031746, 0.00031746031],
+    8: [-8.541666666666666, 1.6, -0.2, 0.025396825396825, -0.001785714285714],
+    6: [-8.16666666, 1.5, -0.15, 0.01111111],
+    4: [-7.5, 1.3333333, -0.08333333],
+    2: [-6.0, 1.0],
+}
+
+
+@pytest.mark.parametrize("order", sorted(REFERENCE_TABLES))
+def test_ccs_tables(order):
+    ref = REFERENCE_TABLES[order]
+    # The table labelled "order 14" lists c0..c8, i.e. the 8-point
+    # (order-16) central stencil; every other label matches its length.
+    n_pts = len(ref) - 1
+    assert n_pts == (8 if order == 14 else order // 2)
+    ours = central_second_derivative(n_pts)
+    assert len(ours) == len(ref)
+    for a, r in zip(ours, ref):
+        assert abs(a - r) <= 2e-7 * max(1.0, abs(r)), (order, a, r)
+
+
+def test_blend_is_order2_order4_mix():
+    # FSLBM.cu:231 equals (1-b) * order 2 + b * order 4
+    o2, o4 = central_second_derivative(1) + [0.0], central_second_derivative(2)
+    for b in (0.0, 0.25, 1.0):
+        mix = [(1 - b) * x + b * y for x, y in zip(o2, o4)]
+        ref = [-6.0 - 1.5 * b, 1.0 + b / 3, -b / 12]
+        assert np.allclose(mix, ref, atol=1e-15)
+
+
+# ----------------------------------------------------------------------
+# padded (reference) layout == folded layout
+# ----------------------------------------------------------------------
+def padded_from_folded(st, mask_f, d2, pin):
+    nz, ny, nx = st[0].shape
+    lz = nz + 2 * d2
+    zs = (np.arange(lz) - d2) % nz
+    jump = np.where(np.arange(lz) < d2, pin, np.where(np.arange(lz) >= lz - d2, -pin, 0.0))
+    out = [s[zs].copy() for s in st]
+    out[0] = out[0] + jump[:, None, None]
+    wall = mask_f[zs] != 0
+    for f in range(4):
+        out[f][wall] = 0.0
+    return out
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-15), (np.float32, 2e-7)])
+def test_padded_equals_folded(orc, dtype, tol):
+    d2, diam, L = 6, 12, 20
+    padded_mask, npad = inputs.pipe_m_mask(diam, L, d2)
+    # pipe.m puts the padding inside the length: lz = L + 2*d2 ... fold it
+    folded_mask = inputs.fold_mask(padded_mask, d2)
+    nz = folded_mask.shape[0]
+    assert nz == L
+    r = np.random.default_rng(11)
+    shp = folded_mask.shape
+    st = [0.05 + 0.01 * r.standard_normal(shp)] + [0.02 * r.standard_normal(shp) for _ in range(3)]
+    st = [np.where(folded_mask == 0, s, 0.0) for s in st]
+    pin, p2 = 2.5e-3, 1.7e-3
+    prm = orc.params(tblend=0.9, pin=pin, nu=0.015, blend=0.4, d1=2)
+    pad = padded_from_folded(st, folded_mask, d2, pin)
+    out_p = orc.step_padded(pad, prm, padded_mask, d2, p2=p2, dtype=dtype)
+    out_f = orc.step_folded(st, prm, mask=folded_mask, gauge=p2, dtype=dtype)
+    exp = padded_from_folded([np.asarray(o, dtype=np.float64) for o in out_f], folded_mask, d2, pin)
+    for f in range(4):
+        a = np.asarray(out_p[f], dtype=np.float64)
+        assert np.abs(a - exp[f]).max() <= tol * 4, f
+
+
+def test_pipe_m_geometry():
+    m, npad = inputs.pipe_m_mask(100, 300, 10)
+    assert m.shape == (320, 120, 120)
+    assert npad == 7843 * 20  # 7843 fluid cells per cross-section
+    fl = np.argwhere(m[100] == 0)
+    assert fl[:, 1].min() == 10 and fl[:, 1].max() == 109  # x range
+    assert (m[:10] != 0).all() and (m[310:] != 0).all()
+    assert set(np.unique(m[:10])) == {1, 125}
+    f = inputs.fold_mask(m, 10)
+    assert f.shape == (300, 120, 120) and (f[0] == 0).sum() == 7843
+
+
+def test_fold_mask_rejects_inconsistent_padding():
+    m, _ = inputs.pipe_m_mask(10, 12, 3)
+    bad = m.copy()
+    bad[0, 8, 8] = 125 if bad[0, 8, 8] == 1 else 1
+    with pytest.raises(Exception):
+        inputs.fold_mask(bad, 3)
+
+
+# ----------------------------------------------------------------------
+# monitoring and statistics
+# ----------------------------------------------------------------------
+def test_monitor_outlet_is_image_of_inlet(orc):
+    d2, diam, L = 5, 10, 16
+    pm, npad = inputs.pipe_m_mask(diam, L, d2)
+    fm = inputs.fold_mask(pm, d2)
+    r = np.random.default_rng(2)
+    st = [np.where(fm == 0, 0.1 + 0.01 * r.standard_normal(fm.shape), 0.0) for _ in range(4)]
+    pin = 3e-3
+    pad = padded_from_folded(st, fm, d2, pin)
+    l1 = npad / (2 * d2)
+    q1, q2, p1, p2 = orc.monitor_padded(pad, d2, l1)
+    assert abs(q1 - q2) < 1e-15
+    assert abs((p1 - p2) - pin) < 1e-15
+    assert abs(q1 - st[3][0][fm[0] == 0].sum() / l1) < 1e-15
+
+
+def test_stats_hand_sequence(orc):
+    s = orc.Stats(1)
+    # samples: p = 1, 3, 5; ux = 1, 3, 5; uy = 2, 2, 8; uz = 0, 1, 0
+    for p, ux, uy, uz in [(1, 1, 2, 0), (3, 3, 2, 1), (5, 5, 8, 0)]:
+        s.accumulate([[p], [ux], [uy], [uz]], delta=2)
+    acc, cnt, cnt2 = s.read()
+    assert (cnt, cnt2) == (3, 3)
+    assert acc[0, 0] == 9 and acc[1, 0] == 9 and acc[2, 0] == 12 and acc[3, 0] == 1
+    # |dev| against the running mean including the sample: 0, 1, 2
+    assert acc[4, 0] == 3 and acc[5, 0] == 3
+    # uy: means 2, 2, 4 -> dev 0, 0, -4
+    assert acc[6, 0] == 4
+    # uz: means 0, 0.5, 1/3 -> dev 0, -0.5, 1/3
+    assert abs(acc[7, 0] - (0.5 + 1 / 3)) < 1e-15
+    assert acc[8, 0] == 8            # ux*uy: 0 + 0 + (-2)(-4)
+    assert abs(acc[9, 0] - (0 + (-1) * (-0.5) + (-2) * (1 / 3))) < 1e-15
+    assert abs(acc[10, 0] - (0 + (-0.5) * 0 + (1 / 3) * (-4))) < 1e-15
+
+
+def test_stats_delta_resets(orc):
+    s = orc.Stats(2)
+    s.accumulate([[1, 1], [2, 2], [3, 3], [4, 4]], delta=2)
+    s.accumulate([[3, 3], [2, 2], [3, 3], [4, 4]], delta=2)
+    s.accumulate([[5, 5], [2, 2], [3, 3], [4, 4]], delta=1)  # second-order reset
+    acc, cnt, cnt2 = s.read()
+    assert (cnt, cnt2) == (3, 1)
+    assert acc[0, 0] == 9 and acc[4, 0] == 2  # |3 - 5| only
+    s.accumulate([[7, 7], [2, 2], [3, 3], [4, 4]], delta=0)  # full reset
+    acc, cnt, cnt2 = s.read()
+    assert (cnt, cnt2) == (1, 1) and acc[0, 0] == 7 and acc[4, 0] == 0
