@@ -3,7 +3,8 @@
 Workload (default): the turbulent-pipe configuration, 256x256x1024 lattice,
 circular wall of radius 120 centred in the cross-section, z periodic with
 the pressure jump pin, FP32, Poiseuille + 5% seeded noise initial state
-(synthetic; Blasius-estimated pin for bulk Re ~ 5300 at nu = 0.01).
+(synthetic; bulk speed 0.05 lattice units, nu = 0.05*2R/5300 for bulk
+Re ~ 5300, Blasius-estimated pin).
 One step = one FSLBM time step over all 67,108,864 lattice cells.
 
 Printed: one JSON line with MLUPS (all lattice cells, the paper's
@@ -101,7 +102,8 @@ def main() -> int:
     nx = ny = args.nx
     nz = args.nz
     R = args.radius
-    nu, re = 0.01, 5300.0
+    re, ubar = 5300.0, 0.05
+    nu = ubar * 2 * R / re
     mask, st, pin, nf = build_pipe(nx, ny, nz, R, nu, re, 0.05, 3)
     prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=0.0)
     kern = {"auto": KERNEL_AUTO, "plain": KERNEL_PLAIN, "tiled": KERNEL_TILED}[args.kernel]

@@ -33,6 +33,10 @@ struct fslbm_ctx {
     int cur = 0;
     uint8_t *mask = nullptr;   // device mask (nullptr: periodic box)
     std::vector<uint8_t> hmask;
+    bool z_invariant = false;  // every z plane of the mask is the same
+    uint8_t *mask2d = nullptr; // that plane (device), for the tiled kernel
+    void *omega = nullptr;     // per-column wall weights (device)
+    int chunk = 0;             // tiled kernel z planes per work item
     double *partials = nullptr;  // reduction scratch
     int npart = 1024;
     double *red = nullptr;       // 8 doubles of results
@@ -240,10 +244,50 @@ int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
     return FSLBM_OK;
 }
 
+bool tiled_ok(const fslbm_ctx *c, int d1)
+{
+    return tiled_supported(c->g, c->precision, d1, c->z_invariant);
+}
+
 bool use_tiled(const fslbm_ctx *c)
 {
     if (c->kernel == FSLBM_KERNEL_PLAIN) return false;
-    return tiled_supported(c->g, c->precision, c->prm.d1);
+    return tiled_ok(c, c->prm.d1);
+}
+
+// Wall weights of a z-invariant mask, per fluid column (x, y):
+// omega0 = sum of W_e, omegax = sum of W_e cx, omegay = sum of W_e cy over
+// the neighbours x+e that are walls (or outside the box).  The z sums of
+// w(cz) are 1 and of w(cz) cz are 0, so omega_z = 0.  These restore the
+// reference's wall rule (own pressure in place of a wall neighbour's,
+// FSLBM.cu:137-143) in the separable predictor.
+template <typename T>
+std::vector<T> wall_weights(int nx, int ny, const uint8_t *m2)
+{
+    const double w[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    const size_t nxy = (size_t)nx * ny;
+    std::vector<T> om(3 * nxy, T(0));
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            const size_t c = x + (size_t)nx * y;
+            if (m2[c] != 0) continue;
+            double o0 = 0, ox = 0, oy = 0;
+            for (int cy = -1; cy <= 1; cy++)
+                for (int cx = -1; cx <= 1; cx++) {
+                    const int xx = x + cx, yy = y + cy;
+                    const bool wall = xx < 0 || yy < 0 || xx >= nx || yy >= ny ||
+                                      m2[xx + (size_t)nx * yy] != 0;
+                    if (!wall) continue;
+                    const double W = w[cx + 1] * w[cy + 1];
+                    o0 += W;
+                    ox += W * cx;
+                    oy += W * cy;
+                }
+            om[c] = (T)o0;
+            om[nxy + c] = (T)ox;
+            om[2 * nxy + c] = (T)oy;
+        }
+    return om;
 }
 
 int launch_one_step(fslbm_ctx *c, double gauge)
@@ -262,11 +306,12 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     const void *in = c->state[c->cur];
     void *out = c->state[c->cur ^ 1];
     hipError_t e;
-    if (c->kernel == FSLBM_KERNEL_TILED && !tiled_supported(c->g, c->precision, c->prm.d1))
+    if (c->kernel == FSLBM_KERNEL_TILED && !tiled_ok(c, c->prm.d1))
         return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
-    if (use_tiled(c))
-        e = launch_step_tiled(c->g, c->precision, in, out, c->mask, a, c->stream);
-    else
+    if (use_tiled(c)) {
+        TiledAux aux{c->mask2d, c->omega, c->chunk, c->stream};
+        e = launch_step_tiled(c->g, c->precision, in, out, aux, a);
+    } else
         e = launch_step_plain(c->g, c->precision, in, out, c->mask, a, c->stream);
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
@@ -305,6 +350,8 @@ void fslbm_destroy(fslbm_t c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : c->state) if (p) (void)hipFree(p);
     if (c->mask) (void)hipFree(c->mask);
+    if (c->mask2d) (void)hipFree(c->mask2d);
+    if (c->omega) (void)hipFree(c->omega);
     if (c->partials) (void)hipFree(c->partials);
     if (c->red) (void)hipFree(c->red);
     if (c->hred) (void)hipHostFree(c->hred);
@@ -397,6 +444,25 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
     if (ok && mask) {
         ok = alloc((void **)&c->mask, (size_t)N) &&
              hipMemcpy(c->mask, mask, (size_t)N, hipMemcpyHostToDevice) == hipSuccess;
+        const size_t nxy = (size_t)nx * ny;
+        c->z_invariant = true;
+        for (int z = 1; z < nz && c->z_invariant; z++)
+            c->z_invariant = std::memcmp(mask, mask + z * nxy, nxy) == 0;
+        if (ok && c->z_invariant) {
+            ok = alloc((void **)&c->mask2d, nxy) &&
+                 hipMemcpy(c->mask2d, mask, nxy, hipMemcpyHostToDevice) == hipSuccess;
+            if (ok && precision == FSLBM_F64) {
+                auto om = wall_weights<double>(nx, ny, mask);
+                ok = alloc(&c->omega, om.size() * sizeof(double)) &&
+                     hipMemcpy(c->omega, om.data(), om.size() * sizeof(double),
+                               hipMemcpyHostToDevice) == hipSuccess;
+            } else if (ok) {
+                auto om = wall_weights<float>(nx, ny, mask);
+                ok = alloc(&c->omega, om.size() * sizeof(float)) &&
+                     hipMemcpy(c->omega, om.data(), om.size() * sizeof(float),
+                               hipMemcpyHostToDevice) == hipSuccess;
+            }
+        }
     }
     if (ok) {
         ok = hipMemsetAsync(c->state[0], 0, sbytes, c->stream) == hipSuccess &&
@@ -439,10 +505,24 @@ int fslbm_set_kernel(fslbm_t c, int variant)
     if (!c) return FSLBM_EINVAL;
     if (variant < FSLBM_KERNEL_AUTO || variant > FSLBM_KERNEL_TILED)
         return fail(c, FSLBM_EINVAL, "unknown kernel variant %d", variant);
-    if (variant == FSLBM_KERNEL_TILED && !tiled_supported(c->g, c->precision, c->prm.d1))
+    if (variant == FSLBM_KERNEL_TILED && !tiled_ok(c, c->prm.d1))
         return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
     c->kernel = variant;
     return FSLBM_OK;
+}
+
+int fslbm_set_tile_chunk(fslbm_t c, int chunk)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (chunk < 0) return fail(c, FSLBM_EINVAL, "chunk must be >= 0");
+    c->chunk = chunk;
+    return FSLBM_OK;
+}
+
+int fslbm_get_tile_chunk(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    return c->chunk > 0 ? std::min(c->chunk, c->g.nz) : default_chunk(c->g);
 }
 
 int fslbm_get_kernel(fslbm_t c)

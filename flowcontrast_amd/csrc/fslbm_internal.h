@@ -54,11 +54,20 @@ size_t storage_bytes(int precision);  // bytes per stored value
 hipError_t launch_step_plain(const Geo &g, int precision, const void *in,
                              void *out, const uint8_t *mask,
                              const StepArgs &a, hipStream_t s);
-// Returns hipErrorNotSupported when the tiled kernel cannot take the case.
+// Extra inputs of the tiled kernel: the z-invariant mask as one plane and
+// the per-column wall weights omega (3 planes of nx*ny values in the
+// arithmetic type: sum over wall neighbours of W_e, W_e cx, W_e cy).
+struct TiledAux {
+    const uint8_t *mask2d;
+    const void *omega;
+    int chunk;  // z planes per work item (0: default)
+    hipStream_t stream;
+};
 hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
-                             void *out, const uint8_t *mask,
-                             const StepArgs &a, hipStream_t s);
-bool tiled_supported(const Geo &g, int precision, int d1);
+                             void *out, const TiledAux &aux,
+                             const StepArgs &a);
+bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
+int default_chunk(const Geo &g);
 
 // Whole-field reductions (8 doubles, see fslbm_reduce) into dst (device).
 hipError_t launch_reduce(const Geo &g, int precision, const void *state,

@@ -1,14 +1,552 @@
-// Tiled z-marching FSLBM step kernel (placeholder until written).
+// Tiled, z-marching FSLBM step kernel for gfx950.
+//
+// Same update as the plain kernel (FSLBM.cu:130-303 semantics), organised
+// for HBM bandwidth:
+//
+// * One workgroup owns a 64 x 8 (x, y) tile and marches over a chunk of
+//   z planes.  Every state value is read from HBM once per step (halo
+//   re-reads hit L2) and written once.
+// * The 27-point D3Q27 predictor is evaluated as three separable 3-tap
+//   passes.  D3Q27 weights are products w(cx) w(cy) w(cz) with
+//   w(0) = 2/3, w(+-1) = 1/6, and the equilibrium
+//     g_e = A + 3 e.u + 4.5 (e.u)^2,   A = p - 1.5 |u|^2
+//   is a quadratic polynomial in e, so sum_e W_e g_e(x+e) (1, e) becomes:
+//     z pass  (registers, own column): 12 coefficients of a polynomial
+//             in (cx, cy) from the planes z-1, z, z+1;
+//     x pass  (LDS): each column publishes that polynomial evaluated at
+//             cx = +1 and cx = -1 (10 values), neighbours gather;
+//     y pass  (LDS): each row publishes the cy = +1 / -1 values of the
+//             three remaining polynomials (6 values).
+//   Weights are carried unnormalised (x6 per axis) and scaled by 1/216
+//   at the end.  Wall cells hold zero state, so they add nothing to the
+//   passes; the reference's wall rule (own pressure in place of a wall
+//   neighbour's, FSLBM.cu:137-143) is added back as p_self * omega with
+//   omega = sum over wall neighbours of W_e (1, e), precomputed per
+//   column for z-invariant masks.
+// * The CCS corrector reads its x/y arms from a double-buffered LDS tile
+//   of the plane (halo 2*d1, clamped or wrapped like the reference's
+//   texture reads) and its z arms from a register queue that runs
+//   several planes ahead of the output plane (loads in flight).
+// * Workgroups are mapped XCD by XCD: the 8 XCDs each take a contiguous
+//   run of work items (z-chunk major), so tiles that share halos share
+//   an L2.
+//
+// Roles of the 704 threads (11 waves):
+//   waves 0-7  interior columns (64 x 8): everything
+//   waves 8-9  halo rows y0-1 and y0+8: z pass, x pass, y messages
+//   wave 10    halo columns x0-1 and x0+64 (20 lanes): z pass, x messages
+
 #include "fslbm_internal.h"
+#include "fslbm_store.h"
 
 namespace fslbm {
 
-bool tiled_supported(const Geo &, int, int) { return false; }
+namespace {
 
-hipError_t launch_step_tiled(const Geo &, int, const void *, void *,
-                             const uint8_t *, const StepArgs &, hipStream_t)
+constexpr int TX = 64, TY = 8;
+constexpr int RX = TX + 2, RY = TY + 2;
+constexpr int NI = TX * TY;       // interior threads
+constexpr int NH = 2 * TX;        // halo-row threads
+constexpr int NS = 2 * RY;        // halo-column threads
+constexpr int NT = 704;           // 11 waves
+
+template <typename R>
+struct Msg {
+    R A, ux, uy, xx, yy, xy, aP, aM, bP, bM, cP, cM;
+};
+
+template <typename R>
+__device__ __forceinline__ Msg<R> make_msg(R p, R ux, R uy, R uz)
 {
-    return hipErrorNotSupported;
+    Msg<R> m;
+    const R uxx = ux * ux, uyy = uy * uy, uzz = uz * uz;
+    m.A = p - R(1.5) * (uxx + uyy + uzz);
+    m.ux = ux;
+    m.uy = uy;
+    m.xx = uxx;
+    m.yy = uyy;
+    m.xy = ux * uy;
+    const R uxz = ux * uz, uyz = uy * uz;
+    const R t = m.A + R(4.5) * uzz;
+    m.aP = t + R(3) * uz;
+    m.aM = t - R(3) * uz;
+    m.bP = ux + R(3) * uxz;
+    m.bM = ux - R(3) * uxz;
+    m.cP = uy + R(3) * uyz;
+    m.cM = uy - R(3) * uyz;
+    return m;
+}
+
+template <typename R>
+__device__ __forceinline__ Msg<R> zero_msg()
+{
+    Msg<R> m;
+    m.A = m.ux = m.uy = m.xx = m.yy = m.xy = R(0);
+    m.aP = m.aM = m.bP = m.bM = m.cP = m.cM = R(0);
+    return m;
+}
+
+// z pass over planes z-1 (mm), z (m0), z+1 (mp) and the x messages.
+template <typename R>
+struct XMsg {
+    R v[10];      // P0+, P0-, P1+, P1-, P2, Z0+, Z0-, Z1+, Z1-, Z2
+    R cA, cC, dA, dC;
+};
+
+template <typename R>
+__device__ __forceinline__ XMsg<R> zpass(const Msg<R> &mm, const Msg<R> &m0,
+                                         const Msg<R> &mp)
+{
+    const R Ca = R(4) * m0.A + (mp.aP + mm.aM), Da = mp.aP - mm.aM;
+    const R Cb = R(4) * m0.ux + (mp.bP + mm.bM), Db = mp.bP - mm.bM;
+    const R Cc = R(4) * m0.uy + (mp.cP + mm.cM), Dc = mp.cP - mm.cM;
+    const R Cd = R(4) * m0.xx + (mp.xx + mm.xx), Dd = mp.xx - mm.xx;
+    const R Ce = R(4) * m0.yy + (mp.yy + mm.yy), De = mp.yy - mm.yy;
+    const R Ch = R(4) * m0.xy + (mp.xy + mm.xy), Dh = mp.xy - mm.xy;
+    XMsg<R> x;
+    const R t = Ca + R(4.5) * Cd;
+    x.v[0] = t + R(3) * Cb;
+    x.v[1] = t - R(3) * Cb;
+    const R c3 = R(3) * Cc;
+    x.v[2] = c3 + R(9) * Ch;
+    x.v[3] = c3 - R(9) * Ch;
+    x.v[4] = R(4.5) * Ce;
+    const R s = Da + R(4.5) * Dd;
+    x.v[5] = s + R(3) * Db;
+    x.v[6] = s - R(3) * Db;
+    const R d3 = R(3) * Dc;
+    x.v[7] = d3 + R(9) * Dh;
+    x.v[8] = d3 - R(9) * Dh;
+    x.v[9] = R(4.5) * De;
+    x.cA = Ca;
+    x.cC = c3;
+    x.dA = Da;
+    x.dC = d3;
+    return x;
+}
+
+template <typename R>
+struct Vec3 {
+    R x, y, z;
+};
+
+// LDS cell of the corrector tile: float4 for FP32 (one 16-byte read),
+// three doubles for FP64.
+template <typename R>
+struct FdCell;
+template <>
+struct alignas(16) FdCell<float> {
+    float x, y, z, w;
+};
+template <>
+struct FdCell<double> {
+    double x, y, z;
+};
+
+template <typename R>
+struct TiledParams {
+    int nx, ny, nz, nxy;
+    int ntx, nty, chunk, nchunks, nwork, per_xcd;
+    R pin, gauge;
+    R c0, c1, c2, g, tb, cc;
+    Quant q;
+};
+
+__device__ __forceinline__ int wrapi(int v, int n)
+{
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+__device__ __forceinline__ int clampi(int v, int n)
+{
+    return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+// Periodic index for set-up coordinates, which may lie several periods
+// away when the box is narrower than the tile.
+__device__ __forceinline__ int modp(int v, int n)
+{
+    const int r = v % n;
+    return r < 0 ? r + n : r;
+}
+
+constexpr int lcm3(int q) { return q % 3 == 0 ? q : 3 * q; }
+
+// Planes of look-ahead beyond the corrector reach, chosen so the queue
+// length 4*d1 + 1 + LA has a small common multiple with 3 (static slots
+// of the three-plane message queue after unrolling).
+template <int D1> struct Ahead;
+template <> struct Ahead<1> { static constexpr int LA = 1; };  // queue 6
+template <> struct Ahead<2> { static constexpr int LA = 3; };  // queue 12
+
+template <int MODE, bool PERIODIC, int D1>
+__global__ __launch_bounds__(NT) void k_step_tiled(
+    const typename Store<MODE>::S *__restrict__ in,
+    typename Store<MODE>::S *__restrict__ out,
+    const uint8_t *__restrict__ mask2d,
+    const typename Store<MODE>::R *__restrict__ omega,
+    TiledParams<typename Store<MODE>::R> P)
+{
+    using ST = Store<MODE>;
+    using R = typename ST::R;
+    using S = typename ST::S;
+    constexpr int HF = 2 * D1;              // corrector reach
+    constexpr int LA = Ahead<D1>::LA;       // extra look-ahead planes
+    constexpr int QL = 2 * HF + 1 + LA;     // planes zo-HF .. zo+HF+LA
+    constexpr int U = lcm3(QL);             // unroll: static queue slots
+    static_assert(U % 2 == 0, "two-deep prefetch slots need an even unroll");
+    constexpr int FX = TX + 2 * HF, FY = TY + 2 * HF;
+
+    __shared__ R xmsg[10][RY][RX];
+    __shared__ R ymsg[6][RY][TX];
+    __shared__ FdCell<R> fd[2][FY][FX];
+
+    // ---- work item (XCD-major order) -------------------------------------
+    const int bid = blockIdx.x;
+    const int w = (bid & 7) * P.per_xcd + (bid >> 3);
+    if (w >= P.nwork) return;
+    const int ntile = P.ntx * P.nty;
+    const int chunk = w / ntile;
+    const int tile = w - chunk * ntile;
+    const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int zs = chunk * P.chunk;
+    const int ze = min(zs + P.chunk, P.nz);
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
+    const int N = nxy * nz;
+
+    // ---- role and column ---------------------------------------------------
+    const int tid = threadIdx.x;
+    int role;  // 0 interior, 1 halo row, 2 halo column, 3 none
+    int x = 0, y = 0, rx = 0, ry = 0, ix = 0, iy = 0;
+    if (tid < NI) {
+        role = 0;
+        ix = tid & (TX - 1);
+        iy = tid / TX;
+        x = x0 + ix;
+        y = y0 + iy;
+        rx = ix + 1;
+        ry = iy + 1;
+    } else if (tid < NI + NH) {
+        role = 1;
+        const int j = tid - NI;
+        ix = j & (TX - 1);
+        const int hr = j / TX;
+        x = x0 + ix;
+        y = hr ? y0 + TY : y0 - 1;
+        rx = ix + 1;
+        ry = hr ? TY + 1 : 0;
+    } else if (tid < NI + NH + NS) {
+        role = 2;
+        const int k = tid - NI - NH;
+        const int side = k / RY, r = k - side * RY;
+        x = side ? x0 + TX : x0 - 1;
+        y = y0 - 1 + r;
+        rx = side ? TX + 1 : 0;
+        ry = r;
+    } else {
+        role = 3;
+    }
+    const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
+    const int xm = PERIODIC ? modp(x, nx) : clampi(x, nx);
+    const int ym = PERIODIC ? modp(y, ny) : clampi(y, ny);
+    const int col = xm + nx * ym;
+    bool fluid = false;
+    if (role != 3) fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+    R om0 = 0, omx = 0, omy = 0;
+    if (!PERIODIC && role == 0 && fluid) {
+        om0 = omega[col];
+        omx = omega[nxy + col];
+        omy = omega[2 * nxy + col];
+    }
+    const bool store = role == 0 && inside && fluid;
+
+    // ---- corrector halo arms: (fy, fx) tile position and memory column ----
+    int arm_fy = -1, arm_fx = 0, arm_col = 0;
+    if (tid < 2 * HF * TX) {  // y arms: rows above and below the tile
+        const int ar = tid / TX, ac = tid - ar * TX;
+        arm_fy = ar < HF ? ar : TY + ar;
+        arm_fx = HF + ac;
+    } else if (tid >= NI && tid < NI + 2 * HF * TY) {  // x arms
+        const int b = tid - NI;
+        const int br = b / (2 * HF), bc = b - br * (2 * HF);
+        arm_fy = HF + br;
+        arm_fx = bc < HF ? bc : TX + bc;
+    }
+    if (arm_fy >= 0) {
+        const int gx = x0 - HF + arm_fx, gy = y0 - HF + arm_fy;
+        const int ax = PERIODIC ? modp(gx, nx) : clampi(gx, nx);
+        const int ay = PERIODIC ? modp(gy, ny) : clampi(gy, ny);
+        arm_col = ax + nx * ay;
+    }
+
+    const S *__restrict__ Pi = in;
+    const S *__restrict__ Ui = in + N;
+    const S *__restrict__ Vi = in + 2 * N;
+    const S *__restrict__ Wi = in + 3 * N;
+
+    // logical plane -> (physical plane, pressure jump)
+    auto phys = [&](int zl, R &jump) -> int {
+        jump = R(0);
+        if (zl < 0) { jump = P.pin; return zl + nz; }
+        if (zl >= nz) { jump = -P.pin; return zl - nz; }
+        return zl;
+    };
+
+    // register queue of the own column (interior threads): planes
+    // zo-HF .. zo+HF+LA in slots (i + j - HF - LA) mod QL
+    R qp[QL], qx[QL], qy[QL], qz[QL];
+    Msg<R> mq[3];
+    // two-deep prefetch of halo-row / halo-column threads, by plane parity
+    R hp[2] = {0, 0}, hx[2] = {0, 0}, hy[2] = {0, 0}, hz[2] = {0, 0};
+    // two-deep prefetch of the corrector arm, by plane parity
+    R ax_[2] = {0, 0}, ay_[2] = {0, 0}, az_[2] = {0, 0};
+
+#pragma unroll
+    for (int k = 0; k < QL; k++) qp[k] = qx[k] = qy[k] = qz[k] = R(0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) mq[k] = zero_msg<R>();
+
+    const int nit = (ze - zs) + 2 * HF + LA;
+    const int zfirst = zs - 2 * HF - LA;
+
+    for (int ib = 0; ib < nit; ib += U) {
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int i = ib + k;
+            if (i >= nit) break;
+            const int zo = zfirst + i;
+
+            // -- halo threads: message of plane zo+1, then prefetch plane
+            //    zo+3 into the slot it used (same parity) --
+            if (role == 1 || role == 2) {
+                const int sl = (k + 1) & 1;  // parity slot of planes zo+1, zo+3
+                if (zo + 1 >= zs - 1 && zo + 1 <= ze)
+                    mq[(k + 1) % 3] = fluid ? make_msg<R>(hp[sl], hx[sl], hy[sl], hz[sl])
+                                            : zero_msg<R>();
+                const int zl = zo + 3;
+                if (zl >= zs - 1 && zl <= ze) {
+                    R jump;
+                    const int zp = phys(zl, jump);
+                    const int gi = col + nxy * zp;
+                    hp[sl] = ST::ld_p(Pi[gi], zp, P.q) + jump;
+                    hx[sl] = ST::ld_u(Ui[gi], P.q);
+                    hy[sl] = ST::ld_u(Vi[gi], P.q);
+                    hz[sl] = ST::ld_u(Wi[gi], P.q);
+                }
+            }
+            // -- interior threads: load plane zo+HF+LA into the queue --
+            if (role == 0) {
+                R jump;
+                const int zl = zo + HF + LA;
+                const int zp = phys(zl, jump);
+                const int gi = col + nxy * zp;
+                const int s = k % QL;
+                qp[s] = fluid ? ST::ld_p(Pi[gi], zp, P.q) + jump : R(0);
+                qx[s] = ST::ld_u(Ui[gi], P.q);
+                qy[s] = ST::ld_u(Vi[gi], P.q);
+                qz[s] = ST::ld_u(Wi[gi], P.q);
+                // message of plane zo+1 (loaded HF+LA-1 iterations ago)
+                if (zo + 1 >= zs - 1) {
+                    const int s1 = (k + 1 - HF - LA + 2 * QL) % QL;
+                    mq[(k + 1) % 3] = fluid ? make_msg<R>(qp[s1], qx[s1], qy[s1], qz[s1])
+                                            : zero_msg<R>();
+                }
+            }
+            // -- corrector arms: write plane zo, prefetch plane zo+2 --
+            if (arm_fy >= 0 && zo >= zs - 2) {
+                const int t = k & 1;  // parity slot of planes zo and zo+2
+                if (zo >= zs) {
+                    FdCell<R> c;
+                    c.x = ax_[t];
+                    c.y = ay_[t];
+                    c.z = az_[t];
+                    fd[zo & 1][arm_fy][arm_fx] = c;
+                }
+                if (zo + 2 < ze) {
+                    const int gi = arm_col + nxy * wrapi(zo + 2, nz);
+                    ax_[t] = ST::ld_u(Ui[gi], P.q);
+                    ay_[t] = ST::ld_u(Vi[gi], P.q);
+                    az_[t] = ST::ld_u(Wi[gi], P.q);
+                }
+            }
+            if (zo < zs) continue;
+
+            // ================= output plane zo =================
+            const int buf = zo & 1;
+            const int s0 = (k - HF - LA + 2 * QL) % QL;  // queue slot of plane zo
+            if (role == 0) {
+                FdCell<R> c;
+                c.x = qx[s0];
+                c.y = qy[s0];
+                c.z = qz[s0];
+                fd[buf][HF + iy][HF + ix] = c;
+            }
+            XMsg<R> xm_;
+            if (role <= 2) {
+                xm_ = zpass<R>(mq[(k + 2) % 3], mq[k % 3], mq[(k + 1) % 3]);
+#pragma unroll
+                for (int v = 0; v < 10; v++) xmsg[v][ry][rx] = xm_.v[v];
+            }
+            __syncthreads();
+
+            R b0 = 0, g0 = 0, d0 = 0;
+            if (role <= 1) {
+                R e[10], wv[10];
+#pragma unroll
+                for (int v = 0; v < 10; v++) {
+                    e[v] = xmsg[v][ry][rx + 1];
+                    wv[v] = xmsg[v][ry][rx - 1];
+                }
+                b0 = R(4) * xm_.cA + (e[0] + wv[1]);
+                const R b1 = R(4) * xm_.cC + (e[2] + wv[3]);
+                const R b2 = R(4) * xm_.v[4] + (e[4] + wv[4]);
+                g0 = e[0] - wv[1];
+                const R g1 = e[2] - wv[3];
+                const R g2 = e[4] - wv[4];
+                d0 = R(4) * xm_.dA + (e[5] + wv[6]);
+                const R d1_ = R(4) * xm_.dC + (e[7] + wv[8]);
+                const R d2 = R(4) * xm_.v[9] + (e[9] + wv[9]);
+                const R tb_ = b0 + b2, tg = g0 + g2, td = d0 + d2;
+                ymsg[0][ry][ix] = tb_ + b1;
+                ymsg[1][ry][ix] = tb_ - b1;
+                ymsg[2][ry][ix] = tg + g1;
+                ymsg[3][ry][ix] = tg - g1;
+                ymsg[4][ry][ix] = td + d1_;
+                ymsg[5][ry][ix] = td - d1_;
+            }
+            __syncthreads();
+
+            if (role == 0) {
+                const R QPp = ymsg[0][ry + 1][ix], QPm = ymsg[1][ry - 1][ix];
+                const R QXp = ymsg[2][ry + 1][ix], QXm = ymsg[3][ry - 1][ix];
+                const R QZp = ymsg[4][ry + 1][ix], QZm = ymsg[5][ry - 1][ix];
+                constexpr R inv216 = R(1) / R(216);
+                const R ps = qp[s0];
+                R Ps = (R(4) * b0 + (QPp + QPm)) * inv216;
+                R Xs = (R(4) * g0 + (QXp + QXm)) * inv216;
+                R Ys = (QPp - QPm) * inv216;
+                R Zs = (R(4) * d0 + (QZp + QZm)) * inv216;
+                if (!PERIODIC) {
+                    Ps += ps * om0;
+                    Xs += ps * omx;
+                    Ys += ps * omy;
+                }
+                // CCS corrector on the time-n velocity
+                const R ux = qx[s0], uy = qy[s0], uz = qz[s0];
+                const int cy_ = HF + iy, cx_ = HF + ix;
+                const FdCell<R> xp1 = fd[buf][cy_][cx_ + D1], xm1 = fd[buf][cy_][cx_ - D1];
+                const FdCell<R> yp1 = fd[buf][cy_ + D1][cx_], ym1 = fd[buf][cy_ - D1][cx_];
+                const FdCell<R> xp2 = fd[buf][cy_][cx_ + 2 * D1], xm2 = fd[buf][cy_][cx_ - 2 * D1];
+                const FdCell<R> yp2 = fd[buf][cy_ + 2 * D1][cx_], ym2 = fd[buf][cy_ - 2 * D1][cx_];
+                constexpr int B = HF + LA;
+                const int zp1 = (k + D1 - B + 2 * QL) % QL, zm1 = (k - D1 - B + 2 * QL) % QL;
+                const int zp2 = (k + 2 * D1 - B + 2 * QL) % QL, zm2 = (k - 2 * D1 - B + 2 * QL) % QL;
+                const R Lx = P.c0 * ux +
+                             P.c1 * (xp1.x + xm1.x + yp1.x + ym1.x + qx[zp1] + qx[zm1]) +
+                             P.c2 * (xp2.x + xm2.x + yp2.x + ym2.x + qx[zp2] + qx[zm2]);
+                const R Ly = P.c0 * uy +
+                             P.c1 * (xp1.y + xm1.y + yp1.y + ym1.y + qy[zp1] + qy[zm1]) +
+                             P.c2 * (xp2.y + xm2.y + yp2.y + ym2.y + qy[zp2] + qy[zm2]);
+                const R Lz = P.c0 * uz +
+                             P.c1 * (xp1.z + xm1.z + yp1.z + ym1.z + qz[zp1] + qz[zm1]) +
+                             P.c2 * (xp2.z + xm2.z + yp2.z + ym2.z + qz[zp2] + qz[zm2]);
+                if (store) {
+                    const int go = col + nxy * zo;
+                    out[go] = ST::st_p(Ps - P.gauge, zo, P.q);
+                    out[N + go] = ST::st_u((Xs + P.g * Lx) * P.tb + ux * P.cc, P.q);
+                    out[2 * N + go] = ST::st_u((Ys + P.g * Ly) * P.tb + uy * P.cc, P.q);
+                    out[3 * N + go] = ST::st_u((Zs + P.g * Lz) * P.tb + uz * P.cc, P.q);
+                }
+            }
+        }
+    }
+}
+
+template <int MODE, bool PERIODIC, int D1>
+hipError_t launch_t(const Geo &g, const void *in, void *out,
+                    const TiledAux &aux,
+                    const TiledParams<typename Store<MODE>::R> &P,
+                    hipStream_t s)
+{
+    using S = typename Store<MODE>::S;
+    using R = typename Store<MODE>::R;
+    const int grid = 8 * P.per_xcd;
+    k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
+        (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_mode(const Geo &g, const void *in, void *out,
+                       const TiledAux &aux, const StepArgs &a,
+                       const StepK<typename Store<MODE>::R> &k)
+{
+    using R = typename Store<MODE>::R;
+    TiledParams<R> P;
+    P.nx = g.nx;
+    P.ny = g.ny;
+    P.nz = g.nz;
+    P.nxy = g.nx * g.ny;
+    P.ntx = (g.nx + TX - 1) / TX;
+    P.nty = (g.ny + TY - 1) / TY;
+    P.chunk = aux.chunk > 0 ? aux.chunk : default_chunk(g);
+    if (P.chunk > g.nz) P.chunk = g.nz;
+    P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
+    P.nwork = P.ntx * P.nty * P.nchunks;
+    P.per_xcd = (P.nwork + 7) / 8;
+    P.pin = k.pin;
+    P.gauge = k.gauge;
+    P.c0 = k.c0;
+    P.c1 = k.c1;
+    P.c2 = k.c2;
+    P.g = k.dtddy * k.m;
+    P.tb = k.c;
+    P.cc = k.cc;
+    P.q = a.q;
+    const bool per = g.periodic != 0;
+    if (a.d1 == 1)
+        return per ? launch_t<MODE, true, 1>(g, in, out, aux, P, aux.stream)
+                   : launch_t<MODE, false, 1>(g, in, out, aux, P, aux.stream);
+    return per ? launch_t<MODE, true, 2>(g, in, out, aux, P, aux.stream)
+               : launch_t<MODE, false, 2>(g, in, out, aux, P, aux.stream);
+}
+
+}  // namespace
+
+int default_chunk(const Geo &g)
+{
+    // Enough work items to give every CU ~4 tiles, chunks of >= 16 planes.
+    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
+    long long c = (g.nz * tiles + 1023) / 1024;
+    if (c < 16) c = 16;
+    if (c > g.nz) c = g.nz;
+    return (int)c;
+}
+
+bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant)
+{
+    (void)precision;
+    if (d1 != 1 && d1 != 2) return false;
+    if (!g.periodic && !z_invariant) return false;
+    // The queue runs up to 2*d1 + 3 planes past the chunk end; a single
+    // periodic wrap must reach every such plane.
+    if (g.nz < 16) return false;
+    return true;
+}
+
+hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
+                             void *out, const TiledAux &aux, const StepArgs &a)
+{
+    switch (precision) {
+    case FSLBM_F64:
+        return launch_mode<FSLBM_F64>(g, in, out, aux, a, make_consts_f64(a));
+    case FSLBM_F32:
+        return launch_mode<FSLBM_F32>(g, in, out, aux, a, make_consts_f32(a));
+    case FSLBM_F16S:
+        return launch_mode<FSLBM_F16S>(g, in, out, aux, a, make_consts_f32(a));
+    case FSLBM_BF16S:
+        return launch_mode<FSLBM_BF16S>(g, in, out, aux, a, make_consts_f32(a));
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace fslbm

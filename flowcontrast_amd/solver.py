@@ -99,6 +99,13 @@ class Solver:
     def kernel(self) -> int:
         return self._chk(lib().fslbm_get_kernel(self._h))
 
+    def set_tile_chunk(self, chunk: int) -> None:
+        self._chk(lib().fslbm_set_tile_chunk(self._h, int(chunk)))
+
+    @property
+    def tile_chunk(self) -> int:
+        return self._chk(lib().fslbm_get_tile_chunk(self._h))
+
     # -- state ----------------------------------------------------------------
     def set_state(self, p, ux, uy, uz) -> None:
         arrs = [np.ascontiguousarray(a).reshape(-1) for a in (p, ux, uy, uz)]

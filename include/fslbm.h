@@ -90,9 +90,17 @@ const char *fslbm_last_error(fslbm_t ctx);
 int fslbm_set_params(fslbm_t ctx, const fslbm_params *prm);
 int fslbm_get_params(fslbm_t ctx, fslbm_params *prm);
 
-/* Choose the kernel variant (FSLBM_KERNEL_*). */
+/* Choose the kernel variant (FSLBM_KERNEL_*).  get returns the variant
+ * the next step will run (PLAIN or TILED).  The tiled kernel takes d1 of
+ * 1 or 2 and a fully periodic box or a mask that is the same in every z
+ * plane; other cases run the plain kernel. */
 int fslbm_set_kernel(fslbm_t ctx, int variant);
 int fslbm_get_kernel(fslbm_t ctx);
+
+/* Tiled kernel schedule: z planes per work item (0 = default heuristic).
+ * get returns the value in effect. */
+int fslbm_set_tile_chunk(fslbm_t ctx, int chunk);
+int fslbm_get_tile_chunk(fslbm_t ctx);
 
 /* Upload / download the whole state as four SoA arrays of nx*ny*nz values
  * (p, ux, uy, uz).  The uploaded state is taken as the state after a

@@ -214,7 +214,8 @@ def test_full_size_pipe_conserves_subspace(variant):
         got = s.get_state(np.float32)
         r = s.reduce()
     assert r[7] == 0
-    assert np.abs(got[1]).max() < 1e-9 and np.abs(got[2]).max() < 1e-9
+    # FP32 roundoff of sums of 27 terms of size ~p (0.01) and ~|w| (0.05)
+    assert np.abs(got[1]).max() < 1e-6 * 0.05 and np.abs(got[2]).max() < 1e-6 * 0.05
     assert np.abs(got[0] - p).max() < 2e-8
-    # z-invariance of the axial velocity is kept exactly up to roundoff
-    assert np.abs(got[3] - got[3][:1]).max() < 1e-8
+    # z-invariance of the axial velocity is kept up to roundoff
+    assert np.abs(got[3] - got[3][:1]).max() < 1e-6 * 0.05

@@ -1,9 +1,13 @@
 """GPU parity of the HIP step against the CPU oracle, through the C ABI.
 
-Tolerances (max |gpu - oracle| / max |oracle| per field unless noted):
-  FP64 GPU vs FP64 oracle           <= 1e-12 after 100 and 1000 steps
-  FP32 GPU vs FP64 oracle           relative L2 <= 1e-5 (velocity, p)
-  FP32 GPU vs FP32 oracle           relative L2 <= 1e-5
+Tolerances:
+  FP64 GPU vs FP64 oracle   max |gpu - oracle| / max |oracle| <= 1e-12 after
+                            100 and 1000 steps, for p and for the velocity
+                            vector (every component normalised by the
+                            largest velocity component: the noise components
+                            ux, uy are ~1% of uz and decay)
+  FP32 GPU vs FP64 oracle   relative L2 <= 1e-5 on p and on the velocity
+  FP32 GPU vs FP32 oracle   vector (all three components together)
 Every kernel variant that supports the case is checked.
 """
 import math
@@ -27,6 +31,22 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def errors_max(got, ref):
+    """[p error, velocity error], max norms; velocity normalised jointly."""
+    ep = max_rel(got[0], ref[0])
+    umax = max(np.abs(ref[f]).max() for f in (1, 2, 3))
+    eu = max(np.abs(np.asarray(got[f]) - ref[f]).max() for f in (1, 2, 3)) / umax
+    return ep, eu
+
+
+def errors_l2(got, ref):
+    """[p error, velocity error], relative L2 (velocity as one vector)."""
+    ep = rel_l2(got[0], ref[0])
+    num = sum(np.sum((np.asarray(got[f], dtype=np.float64) - ref[f]) ** 2) for f in (1, 2, 3))
+    den = sum(np.sum(np.asarray(ref[f], dtype=np.float64) ** 2) for f in (1, 2, 3))
+    return ep, float(np.sqrt(num / den))
 
 
 def c1_case(blend=0.0):
@@ -67,6 +87,7 @@ def test_c1_pipe_f64_100_steps(orc, variant, blend):
         got = s.get_state()
     for f in range(4):
         assert max_rel(got[f], ref[f]) <= 1e-12, f
+    assert max(errors_max(got, ref)) <= 1e-12
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -92,11 +113,9 @@ def test_c1_pipe_f64_1000_steps_with_gauge(orc, variant):
             assert abs(mon[0] - ref[3][0][fluid0].sum() / fluid0.sum()) < 1e-14
             if block == 1:
                 at100 = s.get_state()
-                for f in range(4):
-                    assert max_rel(at100[f], ref[f]) <= 1e-12
+                assert max(errors_max(at100, ref)) <= 1e-12
         got = s.get_state()
-    for f in range(4):
-        assert max_rel(got[f], ref[f]) <= 1e-12, f
+    assert max(errors_max(got, ref)) <= 1e-12
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -109,9 +128,8 @@ def test_c1_pipe_f32_vs_f64_oracle(orc, variant):
         s.set_state(*[a.astype(np.float32) for a in st])
         s.step(100)
         got = s.get_state()
-    for f in range(4):
-        assert rel_l2(got[f], ref[f]) <= 1e-5, f
-        assert rel_l2(got[f], ref32[f]) <= 1e-5, f
+    assert max(errors_l2(got, ref)) <= 1e-5
+    assert max(errors_l2(got, ref32)) <= 1e-5
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -126,13 +144,8 @@ def test_taylor_green_periodic(orc, variant, precision, tol):
         s.set_state(*st)
         s.step(100)
         got = s.get_state()
-    for f in range(4):
-        err = max_rel(got[f], ref[f]) if precision == "f64" else rel_l2(got[f], ref[f])
-        if f == 3:  # w stays ~0 (roundoff only): compare absolutely
-            bound = 1e-12 if precision == "f64" else 1e-6
-            assert np.abs(got[f]).max() < bound
-            continue
-        assert err <= tol, (f, err)
+    err = errors_max(got, ref) if precision == "f64" else errors_l2(got, ref)
+    assert max(err) <= tol, err
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
@@ -151,8 +164,7 @@ def test_odd_shapes_and_d1(orc, variant):
             s.set_gauge(1e-3)
             s.step(7)
             got = s.get_state()
-        for f in range(4):
-            assert max_rel(got[f], ref[f]) <= 1e-12, (nx, ny, nz, f)
+        assert max(errors_max(got, ref)) <= 1e-12, (nx, ny, nz)
 
 
 def test_plain_and_tiled_agree_f32():
@@ -168,5 +180,4 @@ def test_plain_and_tiled_agree_f32():
             pass
     if len(outs) < 2:
         pytest.skip("only one kernel variant available")
-    for f in range(4):
-        assert rel_l2(outs[0][f], outs[1][f]) <= 1e-6
+    assert max(errors_l2(outs[0], outs[1])) <= 1e-6
