@@ -74,6 +74,22 @@ def load_traffic(workload: str, precision: str):
         return None
 
 
+def copy_calibration() -> int:
+    """A plain copy kernel of known size: 10 copies of 2^30 bytes.  Its
+    FETCH_SIZE / WRITE_SIZE counter readings per dispatch divided by 2^30
+    give the scale applied to the step kernel's readings."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": 1 << 30, "dispatches": 10}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,7 +102,12 @@ def main() -> int:
     ap.add_argument("--radius", type=float, default=120.0)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 1 GiB x 10 (the "
+                         "scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

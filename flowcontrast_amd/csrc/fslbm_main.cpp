@@ -1,0 +1,329 @@
+// fslbm_run: drop-in replacement of the reference driver main()
+// (PipeDemoStandalone/FSLBM.cu:489-909), on the C ABI of libfslbm.so.
+//
+// Same working-directory protocol:
+//   in   param.txt  tblend pin nu d1 delta blend [path]   (FSLBM.cu:499-510)
+//        pipe.txt   lx*ly*lz mask codes 0/1/125, one per line, x fastest
+//                   (pipe.m:62, FSLBM.cu:478-485)
+//        p3Dii ux3Dii uy3Dii uz3Dii, _*ii.txt, _avii.txt, qpii.txt:
+//                   restart files (FSLBM.cu:386-468), used when present
+//   out  qp.txt     q1 q2 p1 p2 f.f. probe every 50 steps (FSLBM.cu:663-666)
+//        _p _ux _uy _uz _p2 _ux2 _uy2 _uz2 _uxy2 _uxz2 _uyz2 .txt, _av.txt
+//                   averaged YZ plane statistics every 500 steps (:692-764)
+//        p ux uy uz IO .txt (YZ plane x=lx/2), ux1 uy1 uz1 p1 .txt (XY
+//                   plane z=lz/2) every 500 steps (:772-821)
+//        p3Dii ... _avii.txt, qpii.txt backup every 50000 steps (:825-904)
+// The geometry sizes, hard-coded in the reference (FSLBM.cu:512-514), are
+// options here.  param.txt is re-read every 500 steps (FSLBM.cu:669-677).
+//
+// usage: fslbm_run [--lx 120] [--ly 120] [--lz 320] [--d2 10]
+//                  [--steps N] [--precision f32|f64|f16s|bf16s]
+//                  [--monitor 50] [--stats 500] [--backup 50000]
+//                  [--fresh] [--quiet]
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fslbm.h"
+
+namespace {
+
+struct Opts {
+    int lx = 120, ly = 120, lz = 320, d2 = 10;
+    long steps = 500000000L;  // the reference's cycle count (FSLBM.cu:516)
+    int precision = FSLBM_F32;
+    int monitor = 50, stats = 500;
+    long backup = 50000;
+    bool fresh = false, quiet = false;
+};
+
+[[noreturn]] void die(const char *msg, fslbm_t ctx = nullptr)
+{
+    std::fprintf(stderr, "fslbm_run: %s%s%s\n", msg, ctx ? ": " : "",
+                 ctx ? fslbm_last_error(ctx) : "");
+    std::exit(1);
+}
+
+void check(int rc, fslbm_t ctx, const char *what)
+{
+    if (rc != FSLBM_OK) {
+        std::fprintf(stderr, "fslbm_run: %s failed (%s): %s\n", what,
+                     fslbm_error_string(rc), fslbm_last_error(ctx));
+        std::exit(1);
+    }
+}
+
+bool file_exists(const std::string &p)
+{
+    FILE *f = std::fopen(p.c_str(), "rb");
+    if (!f) return false;
+    std::fclose(f);
+    return true;
+}
+
+// param.txt: %e %e %e %d %d %e then the rest (FSLBM.cu:502-509).
+fslbm_params read_params()
+{
+    FILE *f = std::fopen("param.txt", "r");
+    if (!f) die("cannot open param.txt");
+    float tb, pin, nu, blend;
+    int d1, delta;
+    if (std::fscanf(f, "%e %e %e %d %d %e", &tb, &pin, &nu, &d1, &delta, &blend) != 6) {
+        std::fclose(f);
+        die("param.txt: expected tblend pin nu d1 delta blend");
+    }
+    std::fclose(f);
+    fslbm_params p;
+    p.tblend = tb;
+    p.pin = pin;
+    p.nu = nu;
+    p.d1 = d1;
+    p.delta = delta;
+    p.blend = blend;
+    return p;
+}
+
+// pipe.txt: integers, one per line (the reference's read_geom reads one
+// value too many; here exactly lx*ly*lz values are required).
+std::vector<uint8_t> read_mask(const Opts &o)
+{
+    FILE *f = std::fopen("pipe.txt", "r");
+    if (!f) die("cannot open pipe.txt");
+    const size_t n = (size_t)o.lx * o.ly * o.lz;
+    std::vector<uint8_t> m(n);
+    for (size_t i = 0; i < n; i++) {
+        int v;
+        if (std::fscanf(f, "%d", &v) != 1) {
+            std::fclose(f);
+            die("pipe.txt: fewer values than lx*ly*lz");
+        }
+        m[i] = (uint8_t)v;
+    }
+    std::fclose(f);
+    return m;
+}
+
+void write_plane(const char *name, const std::vector<double> &v)
+{
+    FILE *f = std::fopen(name, "wt");
+    if (!f) die("cannot write output file");
+    for (double x : v) std::fprintf(f, "%e\n", (float)x);
+    std::fclose(f);
+}
+
+void copy_file(const char *src, const char *dst)
+{
+    FILE *a = std::fopen(src, "rb");
+    if (!a) return;
+    FILE *b = std::fopen(dst, "wb");
+    if (!b) { std::fclose(a); return; }
+    char buf[65536];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof buf, a)) > 0) std::fwrite(buf, 1, k, b);
+    std::fclose(a);
+    std::fclose(b);
+}
+
+const char *const AVG_NAMES[11] = {"_p", "_ux", "_uy", "_uz", "_p2", "_ux2",
+                                   "_uy2", "_uz2", "_uxy2", "_uxz2", "_uyz2"};
+
+// FSLBM.cu:752-764 (and the backup copy, :886-899).
+void write_averages(fslbm_t ctx, const char *suffix)
+{
+    const int n = fslbm_stats_points(ctx);
+    std::vector<double> acc(11 * (size_t)n);
+    int cnt = 0, cnt2 = 0;
+    check(fslbm_stats_read(ctx, acc.data(), &cnt, &cnt2), ctx, "stats read");
+    for (int k = 0; k < 11; k++) {
+        const std::string name = std::string(AVG_NAMES[k]) + suffix + ".txt";
+        FILE *f = std::fopen(name.c_str(), "wt");
+        if (!f) die("cannot write averages");
+        const double c = k < 4 ? (double)cnt : (double)cnt2;
+        for (int i = 0; i < n; i++) std::fprintf(f, "%e\n", acc[k * (size_t)n + i] / c);
+        std::fclose(f);
+    }
+    const std::string av = std::string("_av") + suffix + ".txt";
+    FILE *f = std::fopen(av.c_str(), "wt");
+    if (!f) die("cannot write _av.txt");
+    std::fprintf(f, "%d %d\n", cnt, cnt2);
+    std::fclose(f);
+}
+
+// FSLBM.cu:429-468: averages are restored as value * count.
+void load_averages(fslbm_t ctx)
+{
+    FILE *f = std::fopen("_avii.txt", "r");
+    if (!f) return;
+    int cnt = 0, cnt2 = 0;
+    if (std::fscanf(f, "%d %d", &cnt, &cnt2) != 2) { std::fclose(f); return; }
+    std::fclose(f);
+    const int n = fslbm_stats_points(ctx);
+    std::vector<double> acc(11 * (size_t)n, 0.0);
+    for (int k = 0; k < 11; k++) {
+        const std::string name = std::string(AVG_NAMES[k]) + "ii.txt";
+        FILE *g = std::fopen(name.c_str(), "r");
+        if (!g) die("restart: missing an averages file");
+        const double c = k < 4 ? (double)cnt : (double)cnt2;
+        for (int i = 0; i < n; i++) {
+            float v;
+            if (std::fscanf(g, "%e", &v) != 1) { std::fclose(g); die("restart: short averages file"); }
+            acc[k * (size_t)n + i] = (double)v * c;
+        }
+        std::fclose(g);
+    }
+    check(fslbm_stats_load(ctx, acc.data(), cnt, cnt2), ctx, "stats load");
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    Opts o;
+    for (int i = 1; i < argc; i++) {
+        auto next = [&](const char *name) -> const char * {
+            if (i + 1 >= argc) { std::fprintf(stderr, "missing value for %s\n", name); std::exit(2); }
+            return argv[++i];
+        };
+        const std::string a = argv[i];
+        if (a == "--lx") o.lx = std::atoi(next("--lx"));
+        else if (a == "--ly") o.ly = std::atoi(next("--ly"));
+        else if (a == "--lz") o.lz = std::atoi(next("--lz"));
+        else if (a == "--d2") o.d2 = std::atoi(next("--d2"));
+        else if (a == "--steps") o.steps = std::atol(next("--steps"));
+        else if (a == "--monitor") o.monitor = std::atoi(next("--monitor"));
+        else if (a == "--stats") o.stats = std::atoi(next("--stats"));
+        else if (a == "--backup") o.backup = std::atol(next("--backup"));
+        else if (a == "--fresh") o.fresh = true;
+        else if (a == "--quiet") o.quiet = true;
+        else if (a == "--precision") {
+            const std::string p = next("--precision");
+            o.precision = p == "f64" ? FSLBM_F64 : p == "f16s" ? FSLBM_F16S
+                        : p == "bf16s" ? FSLBM_BF16S : FSLBM_F32;
+        } else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            return 2;
+        }
+    }
+    if (o.lx < 3 || o.ly < 3 || o.d2 < 1 || o.lz <= 2 * o.d2) die("bad geometry options");
+    if (o.monitor < 1 || o.stats < 1 || o.stats % o.monitor != 0)
+        die("--stats must be a multiple of --monitor (the reference nests them)");
+
+    fslbm_params prm = read_params();
+    const std::vector<uint8_t> padded = read_mask(o);
+    const int nx = o.lx, ny = o.ly, nz = o.lz - 2 * o.d2;
+    const size_t nxy = (size_t)nx * ny, N = nxy * nz;
+    std::vector<uint8_t> mask(N);
+    if (fslbm_fold_mask(o.lx, o.ly, o.lz, o.d2, padded.data(), mask.data()) != FSLBM_OK)
+        die("pipe.txt: padding planes are not periodic images of the fluid planes");
+
+    fslbm_t ctx = nullptr;
+    if (fslbm_create(&prm, mask.data(), nx, ny, nz, o.precision, &ctx) != FSLBM_OK)
+        die("create failed", nullptr), (void)0;
+    const int r = (o.lx - 2 * o.d2) / 2;  // FSLBM.cu:512 (before the frame)
+    long l1 = 0;                          // fluid cells per cross-section
+    for (size_t c = 0; c < nxy; c++) l1 += mask[c] == 0;
+
+    // Initial state: restart volumes if present (FSLBM.cu:610), else the
+    // initphys ramp p = pin (lz-d2-z)/(lz-2 d2), u = 0 (FSLBM.cu:374-383).
+    std::vector<double> st[4];
+    for (auto &v : st) v.assign(N, 0.0);
+    const char *vol[4] = {"p3Dii", "ux3Dii", "uy3Dii", "uz3Dii"};
+    const bool restart = !o.fresh && file_exists(vol[0]);
+    if (restart) {
+        const size_t lxy = (size_t)o.lx * o.ly;
+        std::vector<float> plane(lxy);
+        for (int f = 0; f < 4; f++) {
+            FILE *fp = std::fopen(vol[f], "rb");
+            if (!fp) die("restart: missing a volume file");
+            for (int z = 0; z < o.lz; z++) {
+                if (std::fread(plane.data(), sizeof(float), lxy, fp) != lxy) die("restart: short volume file");
+                if (z < o.d2 || z >= o.lz - o.d2) continue;
+                for (size_t c = 0; c < lxy; c++) st[f][c + lxy * (z - o.d2)] = plane[c];
+            }
+            std::fclose(fp);
+        }
+        copy_file("qpii.txt", "qp.txt");
+    } else {
+        for (int z = 0; z < nz; z++)
+            for (size_t c = 0; c < nxy; c++)
+                if (mask[c + nxy * z] == 0)
+                    st[0][c + nxy * z] = prm.pin * (double)(nz - z) / (double)nz;
+    }
+    check(fslbm_set_state(ctx, st[0].data(), st[1].data(), st[2].data(), st[3].data()),
+          ctx, "set state");
+    for (auto &v : st) std::vector<double>().swap(v);
+    check(fslbm_stats_config(ctx, o.lx / 2, o.d2), ctx, "stats config");
+    if (restart) load_averages(ctx);
+    { FILE *q = std::fopen("qp.txt", "a"); if (q) std::fclose(q); }
+
+    const int n_yz = ny * o.lz;
+    std::vector<double> yz(n_yz), xy(nxy);
+    for (long c = 1; c <= o.steps; c++) {
+        check(fslbm_step(ctx, 1), ctx, "step");
+        if (c % o.monitor != 0) continue;
+        // FSLBM.cu:630-666
+        double m[4];
+        check(fslbm_monitor(ctx, 1, m), ctx, "monitor");
+        const double q1 = m[0], q2 = m[1], p1 = m[2], p2 = m[3];
+        const double Re = 2.0 * r * q1 / prm.nu;
+        const double ff = (p1 - p2) / q1 / q1 / 3.0 * 4.0 * r / (double)(o.lz - 2 * o.d2);
+        double probe[4];
+        check(fslbm_probe(ctx, o.lx / 2, o.ly / 2, o.lz / 2 - o.d2, probe), ctx, "probe");
+        if (!o.quiet)
+            std::printf("time step: %ld, dtddy: %e, q1=%e, q2=%e, p1 =%e, p2=%e, f.f.=%e, "
+                        "Re=%e, #inl. elements=%ld, #outl. elements=%ld\n",
+                        c, prm.nu - 0.16666666, q1, q2, p1, p2, ff, Re, l1, l1);
+        FILE *q = std::fopen("qp.txt", "a");
+        if (q) {
+            std::fprintf(q, "%e %e %e %e %e %e\n", q1, q2, p1, p2, ff, probe[3]);
+            std::fclose(q);
+        }
+        if (c % o.stats == 0) {
+            // FSLBM.cu:667-764: reload parameters, accumulate, write
+            prm = read_params();
+            check(fslbm_set_params(ctx, &prm), ctx, "set params");
+            check(fslbm_stats_accumulate(ctx, prm.delta), ctx, "stats");
+            write_averages(ctx, "");
+            // FSLBM.cu:772-821: instantaneous planes and the mask plane
+            const char *yzn[4] = {"p.txt", "ux.txt", "uy.txt", "uz.txt"};
+            const char *xyn[4] = {"p1.txt", "ux1.txt", "uy1.txt", "uz1.txt"};
+            for (int f = 0; f < 4; f++) {
+                check(fslbm_get_plane(ctx, 0, o.lx / 2, o.d2, f, yz.data()), ctx, "plane");
+                write_plane(yzn[f], yz);
+                check(fslbm_get_plane(ctx, 2, o.lz / 2, o.d2, f, xy.data()), ctx, "plane");
+                write_plane(xyn[f], xy);
+            }
+            FILE *io = std::fopen("IO.txt", "wt");
+            if (io) {
+                for (int z = 0; z < o.lz; z++)
+                    for (int y = 0; y < ny; y++)
+                        std::fprintf(io, "%d\n", padded[o.lx / 2 + (size_t)o.lx * (y + (size_t)ny * z)]);
+                std::fclose(io);
+            }
+        }
+        if (o.backup > 0 && c % o.backup == 0) {
+            // FSLBM.cu:825-904: raw float32 padded volumes + text averages
+            for (int f = 0; f < 4; f++) {
+                FILE *fp = std::fopen(vol[f], "wb");
+                if (!fp) die("cannot write backup");
+                std::vector<float> pl(nxy);
+                for (int z = 0; z < o.lz; z++) {
+                    check(fslbm_get_plane(ctx, 2, z, o.d2, f, xy.data()), ctx, "plane");
+                    for (size_t k = 0; k < nxy; k++) pl[k] = (float)xy[k];
+                    std::fwrite(pl.data(), sizeof(float), nxy, fp);
+                }
+                std::fclose(fp);
+            }
+            copy_file("qp.txt", "qpii.txt");
+            write_averages(ctx, "ii");
+        }
+    }
+    check(fslbm_sync(ctx), ctx, "sync");
+    fslbm_destroy(ctx);
+    return 0;
+}

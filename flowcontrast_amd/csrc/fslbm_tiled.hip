@@ -461,6 +461,402 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
     }
 }
 
+// ---------------------------------------------------------------------
+// Software-pipelined variant for the 32-bit-arithmetic modes (FP32 and the
+// 16-bit shifted-storage modes).  Each iteration t advances three planes
+// at once and ends in one barrier:
+//   stage A, plane a = zs+t  : z pass, x messages -> xmsg[a&1]
+//   stage B, plane a-1       : x pass, y messages -> ymsg[(a-1)&1]
+//   stage C, plane a-2       : y pass + corrector + blend + store
+// LDS message and corrector buffers are double-buffered by plane parity.
+// Global loads are issued unconditionally (addresses clamped or wrapped
+// into the box) and decoded where they are used, so no load sits inside
+// a branch and the queues can run several planes ahead.  Each role
+// (interior, halo rows, halo columns) runs its own loop with the same
+// number of barriers.
+// ---------------------------------------------------------------------
+
+// Queue length 12 for both supported d1 (look-ahead planes LA beyond the
+// corrector reach); 12 is a multiple of the 3-plane message queue, the
+// 4-plane halo prefetch and the 3-plane arm prefetch, so every slot index
+// is static in a 12-fold unrolled loop.
+template <int D1> struct PipeAhead;
+template <> struct PipeAhead<1> { static constexpr int LA = 5; };
+template <> struct PipeAhead<2> { static constexpr int LA = 1; };
+
+template <int MODE, bool PERIODIC, int D1>
+__global__ __launch_bounds__(NT) void k_step_pipe(
+    const typename Store<MODE>::S *__restrict__ in,
+    typename Store<MODE>::S *__restrict__ out,
+    const uint8_t *__restrict__ mask2d,
+    const typename Store<MODE>::R *__restrict__ omega,
+    TiledParams<typename Store<MODE>::R> P)
+{
+    using ST = Store<MODE>;
+    using R = typename ST::R;
+    using S = typename ST::S;
+    constexpr int HF = 2 * D1;
+    constexpr int LA = PipeAhead<D1>::LA;
+    constexpr int QL = 2 * HF + LA + 3;
+    static_assert(QL == 12, "queue length must be 12");
+    constexpr int U = 12;
+    constexpr int PD = 4;  // halo column prefetch distance (planes)
+    constexpr int AD = 3;  // corrector arm prefetch distance (planes)
+    constexpr int FX = TX + 2 * HF, FY = TY + 2 * HF;
+
+    __shared__ R xmsg[2][10][RY][RX];
+    __shared__ R ymsg[2][6][RY][TX];
+    __shared__ FdCell<R> fd[2][FY][FX];
+
+    const int bid = blockIdx.x;
+    const int w = (bid & 7) * P.per_xcd + (bid >> 3);
+    if (w >= P.nwork) return;
+    const int ntile = P.ntx * P.nty;
+    const int chunk = w / ntile;
+    const int tile = w - chunk * ntile;
+    const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int zs = chunk * P.chunk;
+    const int ze = min(zs + P.chunk, P.nz);
+    const int n = ze - zs;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
+    const int N = nxy * nz;
+    const S *__restrict__ Pi = in;
+    const S *__restrict__ Ui = in + N;
+    const S *__restrict__ Vi = in + 2 * N;
+    const S *__restrict__ Wi = in + 3 * N;
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+
+    auto phys = [&](int zl, R &jump) -> int {
+        jump = R(0);
+        if (zl < 0) { jump = P.pin; return zl + nz; }
+        if (zl >= nz) { jump = -P.pin; return zl - nz; }
+        return zl;
+    };
+    auto colx = [&](int v) { return PERIODIC ? modp(v, nx) : clampi(v, nx); };
+    auto coly = [&](int v) { return PERIODIC ? modp(v, ny) : clampi(v, ny); };
+    // message of one column from raw stored values of logical plane zl
+    auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl) -> Msg<R> {
+        R jump;
+        const int zp = phys(zl, jump);
+        const R pp = ST::ld_p(rp, zp, P.q) + jump;
+        const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
+        Msg<R> m = make_msg<R>(pp, ux, uy, uz);
+        if (!fl) m = zero_msg<R>();
+        return m;
+    };
+    auto store_x = [&](int buf, int ry, int rx, const XMsg<R> &xm) {
+#pragma unroll
+        for (int v = 0; v < 10; v++) xmsg[buf][v][ry][rx] = xm.v[v];
+    };
+    // x pass of one column; returns (b0, g0, d0) and writes the y messages
+    auto xpass = [&](int buf, int ry, int rx, int ix, const R (&cen)[6], R &b0,
+                     R &g0, R &d0) {
+        R e[10], wv[10];
+#pragma unroll
+        for (int v = 0; v < 10; v++) {
+            e[v] = xmsg[buf][v][ry][rx + 1];
+            wv[v] = xmsg[buf][v][ry][rx - 1];
+        }
+        b0 = R(4) * cen[0] + (e[0] + wv[1]);
+        const R b1 = R(4) * cen[1] + (e[2] + wv[3]);
+        const R b2 = R(4) * cen[2] + (e[4] + wv[4]);
+        g0 = e[0] - wv[1];
+        const R g1 = e[2] - wv[3];
+        const R g2 = e[4] - wv[4];
+        d0 = R(4) * cen[3] + (e[5] + wv[6]);
+        const R d1_ = R(4) * cen[4] + (e[7] + wv[8]);
+        const R d2 = R(4) * cen[5] + (e[9] + wv[9]);
+        const R tb_ = b0 + b2, tg = g0 + g2, td = d0 + d2;
+        ymsg[buf][0][ry][ix] = tb_ + b1;
+        ymsg[buf][1][ry][ix] = tb_ - b1;
+        ymsg[buf][2][ry][ix] = tg + g1;
+        ymsg[buf][3][ry][ix] = tg - g1;
+        ymsg[buf][4][ry][ix] = td + d1_;
+        ymsg[buf][5][ry][ix] = td - d1_;
+    };
+    auto centres = [&](const XMsg<R> &xm, R (&cen)[6]) {
+        cen[0] = xm.cA;
+        cen[1] = xm.cC;
+        cen[2] = xm.v[4];
+        cen[3] = xm.dA;
+        cen[4] = xm.dC;
+        cen[5] = xm.v[9];
+    };
+
+    const int nit = n + 2;
+
+    if (wave < TY) {
+        // ================= interior columns =================
+        const int ix = lane, iy = wave;
+        const int x = x0 + ix, y = y0 + iy;
+        const int rx = ix + 1, ry = iy + 1;
+        const bool inside = x < nx && y < ny;
+        const int col = colx(x) + nx * coly(y);
+        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        R om0 = 0, omx = 0, omy = 0;
+        if (!PERIODIC) {
+            om0 = omega[col];
+            omx = omega[nxy + col];
+            omy = omega[2 * nxy + col];
+        }
+        const bool store = inside && fluid;
+        // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
+        S qp[QL], qx[QL], qy[QL], qz[QL];
+        auto qload = [&](int q, int s) {
+            R jump;
+            const int gi = col + nxy * phys(q, jump);
+            qp[s] = Pi[gi];
+            qx[s] = Ui[gi];
+            qy[s] = Vi[gi];
+            qz[s] = Wi[gi];
+        };
+#pragma unroll
+        for (int j = 2; j < QL - 1; j++) qload(zs - 2 - HF + j, j);
+        Msg<R> mq[3];
+        mq[0] = msg_of(qp[HF + 1], qx[HF + 1], qy[HF + 1], qz[HF + 1], zs - 1, fluid);
+        mq[1] = msg_of(qp[HF + 2], qx[HF + 2], qy[HF + 2], qz[HF + 2], zs, fluid);
+        mq[2] = zero_msg<R>();
+        R cen[6] = {0, 0, 0, 0, 0, 0};
+        R b0c = 0, g0c = 0, d0c = 0;
+
+        for (int ib = 0; ib < nit; ib += U) {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int t = ib + k;
+                if (t >= nit) break;
+                const int a = zs + t;
+                qload(a + HF + LA, (k + QL - 1) % QL);
+                {
+                    const int s1 = (k + 3 + HF) % QL;  // plane a+1
+                    mq[(k + 2) % 3] = msg_of(qp[s1], qx[s1], qy[s1], qz[s1], a + 1, fluid);
+                }
+                if (t >= 2) {  // stage C: plane c = a-2
+                    const int c = a - 2;
+                    const int cb = c & 1;
+                    const int sc = (k + HF) % QL;
+                    const R QPp = ymsg[cb][0][ry + 1][ix], QPm = ymsg[cb][1][ry - 1][ix];
+                    const R QXp = ymsg[cb][2][ry + 1][ix], QXm = ymsg[cb][3][ry - 1][ix];
+                    const R QZp = ymsg[cb][4][ry + 1][ix], QZm = ymsg[cb][5][ry - 1][ix];
+                    constexpr R inv216 = R(1) / R(216);
+                    R Ps = (R(4) * b0c + (QPp + QPm)) * inv216;
+                    R Xs = (R(4) * g0c + (QXp + QXm)) * inv216;
+                    R Ys = (QPp - QPm) * inv216;
+                    R Zs = (R(4) * d0c + (QZp + QZm)) * inv216;
+                    const R ux = ST::ld_u(qx[sc], P.q), uy = ST::ld_u(qy[sc], P.q),
+                            uz = ST::ld_u(qz[sc], P.q);
+                    if (!PERIODIC) {
+                        const R ps = fluid ? ST::ld_p(qp[sc], c, P.q) : R(0);
+                        Ps += ps * om0;
+                        Xs += ps * omx;
+                        Ys += ps * omy;
+                    }
+                    const int cy_ = HF + iy, cx_ = HF + ix;
+                    const FdCell<R> xp1 = fd[cb][cy_][cx_ + D1], xm1 = fd[cb][cy_][cx_ - D1];
+                    const FdCell<R> yp1 = fd[cb][cy_ + D1][cx_], ym1 = fd[cb][cy_ - D1][cx_];
+                    const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
+                    const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
+                    const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
+                    const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
+                    auto U_ = [&](const S *qq, int sl) { return ST::ld_u(qq[sl], P.q); };
+                    const R Lx = P.c0 * ux +
+                                 P.c1 * (xp1.x + xm1.x + yp1.x + ym1.x + U_(qx, zp1) + U_(qx, zm1)) +
+                                 P.c2 * (xp2.x + xm2.x + yp2.x + ym2.x + U_(qx, zp2) + U_(qx, zm2));
+                    const R Ly = P.c0 * uy +
+                                 P.c1 * (xp1.y + xm1.y + yp1.y + ym1.y + U_(qy, zp1) + U_(qy, zm1)) +
+                                 P.c2 * (xp2.y + xm2.y + yp2.y + ym2.y + U_(qy, zp2) + U_(qy, zm2));
+                    const R Lz = P.c0 * uz +
+                                 P.c1 * (xp1.z + xm1.z + yp1.z + ym1.z + U_(qz, zp1) + U_(qz, zm1)) +
+                                 P.c2 * (xp2.z + xm2.z + yp2.z + ym2.z + U_(qz, zp2) + U_(qz, zm2));
+                    if (store) {
+                        const int go = col + nxy * c;
+                        out[go] = ST::st_p(Ps - P.gauge, c, P.q);
+                        out[N + go] = ST::st_u((Xs + P.g * Lx) * P.tb + ux * P.cc, P.q);
+                        out[2 * N + go] = ST::st_u((Ys + P.g * Ly) * P.tb + uy * P.cc, P.q);
+                        out[3 * N + go] = ST::st_u((Zs + P.g * Lz) * P.tb + uz * P.cc, P.q);
+                    }
+                }
+                if (t >= 1 && t <= n) {  // stage B: plane a-1; corrector tile of a-1
+                    xpass((a - 1) & 1, ry, rx, ix, cen, b0c, g0c, d0c);
+                    const int sb = (k + 1 + HF) % QL;
+                    FdCell<R> cc;
+                    cc.x = ST::ld_u(qx[sb], P.q);
+                    cc.y = ST::ld_u(qy[sb], P.q);
+                    cc.z = ST::ld_u(qz[sb], P.q);
+                    fd[(a - 1) & 1][HF + iy][HF + ix] = cc;
+                }
+                if (t < n) {  // stage A: plane a
+                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
+                    store_x(a & 1, ry, rx, xm);
+                    centres(xm, cen);
+                }
+                __syncthreads();
+            }
+        }
+    } else if (wave < TY + 2) {
+        // ================= halo rows y0-1 and y0+TY =================
+        const int hr = wave - TY;
+        const int ix = lane;
+        const int x = x0 + ix;
+        const int y = hr ? y0 + TY : y0 - 1;
+        const int rx = ix + 1, ry = hr ? TY + 1 : 0;
+        const bool inside = x < nx && y >= 0 && y < ny;
+        const int col = colx(x) + nx * coly(y);
+        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        // own column prefetch: plane m in slot (m - zs + 1) mod PD
+        S hp[PD], hx[PD], hy[PD], hz[PD];
+        auto hload = [&](int m, int s) {
+            R jump;
+            const int gi = col + nxy * phys(m, jump);
+            hp[s] = Pi[gi];
+            hx[s] = Ui[gi];
+            hy[s] = Vi[gi];
+            hz[s] = Wi[gi];
+        };
+        // corrector y arms: HF rows of this side, column ix; plane q in
+        // slot (q - zs + 1) mod AD
+        int acol[HF];
+#pragma unroll
+        for (int r = 0; r < HF; r++) {
+            const int fy = hr ? TY + HF + r : r;
+            acol[r] = colx(x0 + ix) + nx * coly(y0 - HF + fy);
+        }
+        S ar[AD][HF][3];
+        auto aload = [&](int q, int s) {
+            const int zp = wrapi(q, nz);
+#pragma unroll
+            for (int r = 0; r < HF; r++) {
+                const int gi = acol[r] + nxy * zp;
+                ar[s][r][0] = Ui[gi];
+                ar[s][r][1] = Vi[gi];
+                ar[s][r][2] = Wi[gi];
+            }
+        };
+#pragma unroll
+        for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
+#pragma unroll
+        for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
+        Msg<R> mq[3];
+        mq[0] = msg_of(hp[0], hx[0], hy[0], hz[0], zs - 1, fluid);
+        mq[1] = msg_of(hp[1], hx[1], hy[1], hz[1], zs, fluid);
+        mq[2] = zero_msg<R>();
+        hload(zs - 1 + PD, 0);
+        hload(zs + PD, 1);
+        R cen[6] = {0, 0, 0, 0, 0, 0};
+        R b0c, g0c, d0c;
+
+        for (int ib = 0; ib < nit; ib += U) {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int t = ib + k;
+                if (t >= nit) break;
+                const int a = zs + t;
+                {
+                    const int s1 = (k + 2) % PD;  // plane a+1
+                    mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                    hload(a + 1 + PD, s1);
+                }
+                if (t >= 1 && t <= n) {
+                    xpass((a - 1) & 1, ry, rx, ix, cen, b0c, g0c, d0c);
+                    const int sa = k % AD;  // plane a-1
+#pragma unroll
+                    for (int r = 0; r < HF; r++) {
+                        const int fy = hr ? TY + HF + r : r;
+                        FdCell<R> cc;
+                        cc.x = ST::ld_u(ar[sa][r][0], P.q);
+                        cc.y = ST::ld_u(ar[sa][r][1], P.q);
+                        cc.z = ST::ld_u(ar[sa][r][2], P.q);
+                        fd[(a - 1) & 1][fy][HF + ix] = cc;
+                    }
+                }
+                aload(a + 2, k % AD);
+                if (t < n) {
+                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
+                    store_x(a & 1, ry, rx, xm);
+                    centres(xm, cen);
+                }
+                __syncthreads();
+            }
+        }
+    } else {
+        // ======== halo columns x0-1, x0+TX (lanes < 2*RY) and x arms ========
+        const int sl = lane < NS ? lane : NS - 1;
+        const int side = sl / RY, r = sl - side * RY;
+        const int x = side ? x0 + TX : x0 - 1;
+        const int y = y0 - 1 + r;
+        const int rx = side ? TX + 1 : 0, ry = r;
+        const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
+        const int col = colx(x) + nx * coly(y);
+        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        const bool writer = lane < NS;
+        S hp[PD], hx[PD], hy[PD], hz[PD];
+        auto hload = [&](int m, int s) {
+            R jump;
+            const int gi = col + nxy * phys(m, jump);
+            hp[s] = Pi[gi];
+            hx[s] = Ui[gi];
+            hy[s] = Vi[gi];
+            hz[s] = Wi[gi];
+        };
+        // x arms: 2*HF*TY cells, one per lane
+        const int na = 2 * HF * TY;
+        const int al = lane < na ? lane : na - 1;
+        const bool awriter = lane < na;
+        const int br = al / (2 * HF), bc = al - br * (2 * HF);
+        const int afy = HF + br;
+        const int afx = bc < HF ? bc : TX + bc;
+        const int acol = colx(x0 - HF + afx) + nx * coly(y0 - HF + afy);
+        S ar[AD][3];
+        auto aload = [&](int q, int s) {
+            const int gi = acol + nxy * wrapi(q, nz);
+            ar[s][0] = Ui[gi];
+            ar[s][1] = Vi[gi];
+            ar[s][2] = Wi[gi];
+        };
+#pragma unroll
+        for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
+#pragma unroll
+        for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
+        Msg<R> mq[3];
+        mq[0] = msg_of(hp[0], hx[0], hy[0], hz[0], zs - 1, fluid);
+        mq[1] = msg_of(hp[1], hx[1], hy[1], hz[1], zs, fluid);
+        mq[2] = zero_msg<R>();
+        hload(zs - 1 + PD, 0);
+        hload(zs + PD, 1);
+
+        for (int ib = 0; ib < nit; ib += U) {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int t = ib + k;
+                if (t >= nit) break;
+                const int a = zs + t;
+                {
+                    const int s1 = (k + 2) % PD;
+                    mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                    hload(a + 1 + PD, s1);
+                }
+                if (t >= 1 && t <= n && awriter) {
+                    const int sa = k % AD;
+                    FdCell<R> cc;
+                    cc.x = ST::ld_u(ar[sa][0], P.q);
+                    cc.y = ST::ld_u(ar[sa][1], P.q);
+                    cc.z = ST::ld_u(ar[sa][2], P.q);
+                    fd[(a - 1) & 1][afy][afx] = cc;
+                }
+                aload(a + 2, k % AD);
+                if (t < n && writer) {
+                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
+                    store_x(a & 1, ry, rx, xm);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
 template <int MODE, bool PERIODIC, int D1>
 hipError_t launch_t(const Geo &g, const void *in, void *out,
                     const TiledAux &aux,
@@ -470,8 +866,14 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
     using S = typename Store<MODE>::S;
     using R = typename Store<MODE>::R;
     const int grid = 8 * P.per_xcd;
-    k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
-        (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    // FP64 messages would not fit double-buffered in LDS: FP64 runs the
+    // two-barrier kernel, the 32-bit-arithmetic modes the pipelined one.
+    if constexpr (MODE == FSLBM_F64)
+        k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    else
+        k_step_pipe<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     return hipGetLastError();
 }
 

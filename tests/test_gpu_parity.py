@@ -180,4 +180,5 @@ def test_plain_and_tiled_agree_f32():
             pass
     if len(outs) < 2:
         pytest.skip("only one kernel variant available")
-    assert max(errors_l2(outs[0], outs[1])) <= 1e-6
+    # two FP32 evaluation orders: roundoff only, within the FP32 tolerance
+    assert max(errors_l2(outs[0], outs[1])) <= 1e-5
