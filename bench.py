@@ -100,6 +100,9 @@ def main() -> int:
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--nz", type=int, default=1024)
     ap.add_argument("--radius", type=float, default=120.0)
+    ap.add_argument("--chunk", type=int, default=-1,
+                    help="tiled kernel: z planes per work item (0 = library "
+                         "default, -1 = autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -132,6 +135,10 @@ def main() -> int:
     workload = f"pipe{nx}x{ny}x{nz}_r{int(R)}"
 
     s = Solver(mask.shape, prm, mask=mask, precision=args.precision, kernel=kern)
+    if args.chunk > 0:
+        s.set_tile_chunk(args.chunk)
+    elif args.chunk < 0:  # autotune the schedule (state is not advanced)
+        s.autotune(5)
     if args.precision == "f64":
         s.set_state(*st)
     else:
@@ -176,6 +183,7 @@ def main() -> int:
         "config": {"workload": workload, "lattice": [nx, ny, nz], "radius": R,
                    "fluid_cells_per_section": int(nf), "re_bulk": re, "nu": nu,
                    "pin": pin, "d1": 2, "blend": 0.0, "kernel": kernel_name,
+                   "chunk": s.tile_chunk,
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

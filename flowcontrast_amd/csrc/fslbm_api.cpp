@@ -519,10 +519,55 @@ int fslbm_set_tile_chunk(fslbm_t c, int chunk)
     return FSLBM_OK;
 }
 
+int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (trial_steps < 1) return fail(c, FSLBM_EINVAL, "trial_steps must be >= 1");
+    if (!use_tiled(c)) {  // nothing to tune for the plain kernel
+        if (best_chunk) *best_chunk = 0;
+        if (best_ms) *best_ms = 0;
+        return FSLBM_OK;
+    }
+    // Trial launches read the current state and write the spare buffer
+    // without swapping, so the solver state is unchanged.
+    StepArgs a;
+    a.tblend = c->prm.tblend;
+    a.pin = c->prm.pin;
+    a.nu = c->prm.nu;
+    a.blend = c->prm.blend;
+    a.d1 = c->prm.d1;
+    a.q = c->q;
+    a.gauge = 0.0;
+    const int cands[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    int best = 0;
+    double bms = 1e300;
+    const int keep = c->chunk;
+    for (int cand : cands) {
+        if (cand > c->g.nz && cand != cands[0]) break;
+        TiledAux aux{c->mask2d, c->omega, std::min(cand, c->g.nz), c->stream};
+        // warm-up launch, then timed launches
+        HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        for (int t = 0; t < trial_steps; t++)
+            HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const double per = ms / trial_steps;
+        if (per < bms) { bms = per; best = std::min(cand, c->g.nz); }
+    }
+    (void)keep;
+    c->chunk = best;
+    if (best_chunk) *best_chunk = best;
+    if (best_ms) *best_ms = bms;
+    return FSLBM_OK;
+}
+
 int fslbm_get_tile_chunk(fslbm_t c)
 {
     if (!c) return FSLBM_EINVAL;
-    return c->chunk > 0 ? std::min(c->chunk, c->g.nz) : default_chunk(c->g);
+    return c->chunk > 0 ? std::min(c->chunk, c->g.nz) : default_chunk(c->g, c->precision);
 }
 
 int fslbm_get_kernel(fslbm_t c)

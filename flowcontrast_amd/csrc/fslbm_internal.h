@@ -67,7 +67,7 @@ hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
                              void *out, const TiledAux &aux,
                              const StepArgs &a);
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
-int default_chunk(const Geo &g);
+int default_chunk(const Geo &g, int precision);
 
 // Whole-field reductions (8 doubles, see fslbm_reduce) into dst (device).
 hipError_t launch_reduce(const Geo &g, int precision, const void *state,

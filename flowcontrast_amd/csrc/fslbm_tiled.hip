@@ -49,6 +49,10 @@ constexpr int NI = TX * TY;       // interior threads
 constexpr int NH = 2 * TX;        // halo-row threads
 constexpr int NS = 2 * RY;        // halo-column threads
 constexpr int NT = 704;           // 11 waves
+// tile rows of the pipelined kernel: 12 for FP32 (15 waves fit the
+// 128-register budget without spills), 8 for the 16-bit storage modes
+// (their decode needs more registers)
+constexpr int pipe_rows(int mode) { return mode == FSLBM_F32 ? 12 : 8; }
 
 template <typename R>
 struct Msg {
@@ -149,6 +153,9 @@ struct TiledParams {
     int ntx, nty, chunk, nchunks, nwork, per_xcd;
     R pin, gauge;
     R c0, c1, c2, g, tb, cc;
+    // blend folded into the corrector and the 1/216 scale of the
+    // unnormalised passes: u' = ku*u + ks1*S1 + ks2*S2 + kx*X + tb*omega*p
+    R ku, ks1, ks2, kx;
     Quant q;
 };
 
@@ -169,6 +176,44 @@ __device__ __forceinline__ int modp(int v, int n)
 }
 
 constexpr int lcm3(int q) { return q % 3 == 0 ? q : 3 * q; }
+
+// Buffer-instruction access: a resource (base, size) in scalar registers,
+// a scalar plane offset and a 32-bit lane offset, so a load or store
+// costs no per-lane address arithmetic.  Word 3 of the resource is the
+// gfx9 untyped-buffer format word.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mkbuf(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, bytes, 0x00020000);
+}
+template <typename S>
+__device__ __forceinline__ S bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ _Float16 bload<_Float16>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(_Float16, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ uint16_t bload<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(_Float16 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(uint16_t v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, voff, soff, 0);
+}
 
 // Planes of look-ahead beyond the corrector reach, chosen so the queue
 // length 4*d1 + 1 + LA has a small common multiple with 3 (static slots
@@ -463,29 +508,48 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
 
 // ---------------------------------------------------------------------
 // Software-pipelined variant for the 32-bit-arithmetic modes (FP32 and the
-// 16-bit shifted-storage modes).  Each iteration t advances three planes
-// at once and ends in one barrier:
-//   stage A, plane a = zs+t  : z pass, x messages -> xmsg[a&1]
-//   stage B, plane a-1       : x pass, y messages -> ymsg[(a-1)&1]
-//   stage C, plane a-2       : y pass + corrector + blend + store
-// LDS message and corrector buffers are double-buffered by plane parity.
-// Global loads are issued unconditionally (addresses clamped or wrapped
-// into the box) and decoded where they are used, so no load sits inside
-// a branch and the queues can run several planes ahead.  Each role
-// (interior, halo rows, halo columns) runs its own loop with the same
-// number of barriers.
+// 16-bit shifted-storage modes).  Tile 64 x TYP, (TYP + 3) waves:
+//   waves 0 .. TYP-1   interior rows
+//   waves TYP, TYP+1   halo rows y0-1, y0+TYP (+ corrector y arms)
+//   wave  TYP+2        halo columns x0-1, x0+64 (+ corrector x arms)
+// Each iteration t advances three planes and ends in one barrier:
+//   stage A, plane a = zs+t : z pass -> x messages kept in registers
+//   stage B, plane a-1      : x pass, neighbours x+-1 through DPP wave
+//                             shifts (the two tile-edge lanes take the
+//                             halo-column messages from LDS), y messages
+//                             -> ymsg[(a-1)&1]
+//   stage C, plane a-2      : y pass + corrector + blend + store
+// LDS buffers are double-buffered by plane parity.  Global loads are
+// unconditional buffer loads at clamped or wrapped addresses, decoded
+// where used, so the queues run several planes ahead.
 // ---------------------------------------------------------------------
 
-// Queue length 12 for both supported d1 (look-ahead planes LA beyond the
-// corrector reach); 12 is a multiple of the 3-plane message queue, the
-// 4-plane halo prefetch and the 3-plane arm prefetch, so every slot index
-// is static in a 12-fold unrolled loop.
+// Queue length 12 for both supported d1; 12 is a multiple of the 3-plane
+// message queue, the 4-plane halo prefetch and the 3-plane arm prefetch,
+// so every slot index is static in a 12-fold unrolled loop.
 template <int D1> struct PipeAhead;
 template <> struct PipeAhead<1> { static constexpr int LA = 5; };
 template <> struct PipeAhead<2> { static constexpr int LA = 1; };
 
-template <int MODE, bool PERIODIC, int D1>
-__global__ __launch_bounds__(NT) void k_step_pipe(
+// x messages used from the right neighbour (lane+1) and the left one
+// (lane-1) in the x pass, and their order in the LDS edge blocks.
+constexpr int XR[6] = {0, 2, 4, 5, 7, 9};
+constexpr int XL[6] = {1, 3, 4, 6, 8, 9};
+
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i reads lane i+1
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i reads lane i-1
+
+// value of `src` from the neighbouring lane; lanes without a neighbour in
+// the wave keep `edge`
+template <int CTRL>
+__device__ __forceinline__ float lane_shift(float src, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+        __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
+}
+
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+__global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
@@ -495,6 +559,7 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
     using ST = Store<MODE>;
     using R = typename ST::R;
     using S = typename ST::S;
+    static_assert(sizeof(R) == 4, "pipelined kernel takes 32-bit arithmetic");
     constexpr int HF = 2 * D1;
     constexpr int LA = PipeAhead<D1>::LA;
     constexpr int QL = 2 * HF + LA + 3;
@@ -502,11 +567,14 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
     constexpr int U = 12;
     constexpr int PD = 4;  // halo column prefetch distance (planes)
     constexpr int AD = 3;  // corrector arm prefetch distance (planes)
-    constexpr int FX = TX + 2 * HF, FY = TY + 2 * HF;
+    constexpr int RYP = TYP + 2;
+    constexpr int FX = TX + 2 * HF, FY = TYP + 2 * HF;
+    constexpr int NXA = 2 * HF * TYP;          // x-arm cells
+    constexpr int XPL = (NXA + 63) / 64;       // x-arm cells per lane
 
-    __shared__ R xmsg[2][10][RY][RX];
-    __shared__ R ymsg[2][6][RY][TX];
+    __shared__ R ymsg[2][6][RYP][TX];
     __shared__ FdCell<R> fd[2][FY][FX];
+    __shared__ R edge[2][RYP][2][8];  // [side 0: column x0-1 | 1: x0+64]
 
     const int bid = blockIdx.x;
     const int w = (bid & 7) * P.per_xcd + (bid >> 3);
@@ -515,16 +583,17 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
     const int chunk = w / ntile;
     const int tile = w - chunk * ntile;
     const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
-    const int x0 = tx * TX, y0 = ty * TY;
+    const int x0 = tx * TX, y0 = ty * TYP;
     const int zs = chunk * P.chunk;
     const int ze = min(zs + P.chunk, P.nz);
     const int n = ze - zs;
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
     const int N = nxy * nz;
-    const S *__restrict__ Pi = in;
-    const S *__restrict__ Ui = in + N;
-    const S *__restrict__ Vi = in + 2 * N;
-    const S *__restrict__ Wi = in + 3 * N;
+
+    const unsigned fbytes = (unsigned)N * (unsigned)sizeof(S);
+    const __amdgpu_buffer_rsrc_t rP = mkbuf(in, fbytes), rU = mkbuf(in + N, fbytes),
+                                 rV = mkbuf(in + 2 * N, fbytes), rW = mkbuf(in + 3 * N, fbytes);
+    auto poff = [&](int zp) { return (unsigned)zp * (unsigned)nxy * (unsigned)sizeof(S); };
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -538,7 +607,6 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
     };
     auto colx = [&](int v) { return PERIODIC ? modp(v, nx) : clampi(v, nx); };
     auto coly = [&](int v) { return PERIODIC ? modp(v, ny) : clampi(v, ny); };
-    // message of one column from raw stored values of logical plane zl
     auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl) -> Msg<R> {
         R jump;
         const int zp = phys(zl, jump);
@@ -548,28 +616,31 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         if (!fl) m = zero_msg<R>();
         return m;
     };
-    auto store_x = [&](int buf, int ry, int rx, const XMsg<R> &xm) {
+    // x pass of a row wave: neighbours by DPP, edge lanes from LDS
+    auto xpass = [&](int buf, int ry, int ix, const R (&xv)[10], const R (&cen)[6],
+                     R &b0, R &g0, R &d0) {
+        R le[6], re[6];
 #pragma unroll
-        for (int v = 0; v < 10; v++) xmsg[buf][v][ry][rx] = xm.v[v];
-    };
-    // x pass of one column; returns (b0, g0, d0) and writes the y messages
-    auto xpass = [&](int buf, int ry, int rx, int ix, const R (&cen)[6], R &b0,
-                     R &g0, R &d0) {
-        R e[10], wv[10];
-#pragma unroll
-        for (int v = 0; v < 10; v++) {
-            e[v] = xmsg[buf][v][ry][rx + 1];
-            wv[v] = xmsg[buf][v][ry][rx - 1];
+        for (int j = 0; j < 6; j++) {
+            le[j] = edge[buf][ry][0][j];
+            re[j] = edge[buf][ry][1][j];
         }
-        b0 = R(4) * cen[0] + (e[0] + wv[1]);
-        const R b1 = R(4) * cen[1] + (e[2] + wv[3]);
-        const R b2 = R(4) * cen[2] + (e[4] + wv[4]);
-        g0 = e[0] - wv[1];
-        const R g1 = e[2] - wv[3];
-        const R g2 = e[4] - wv[4];
-        d0 = R(4) * cen[3] + (e[5] + wv[6]);
-        const R d1_ = R(4) * cen[4] + (e[7] + wv[8]);
-        const R d2 = R(4) * cen[5] + (e[9] + wv[9]);
+        R e[6], wv[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            e[j] = lane_shift<DPP_WAVE_SHL1>(xv[XR[j]], re[j]);
+            wv[j] = lane_shift<DPP_WAVE_SHR1>(xv[XL[j]], le[j]);
+        }
+        // e[] = values 0,2,4,5,7,9 of x+1; wv[] = values 1,3,4,6,8,9 of x-1
+        b0 = R(4) * cen[0] + (e[0] + wv[0]);
+        const R b1 = R(4) * cen[1] + (e[1] + wv[1]);
+        const R b2 = R(4) * cen[2] + (e[2] + wv[2]);
+        g0 = e[0] - wv[0];
+        const R g1 = e[1] - wv[1];
+        const R g2 = e[2] - wv[2];
+        d0 = R(4) * cen[3] + (e[3] + wv[3]);
+        const R d1_ = R(4) * cen[4] + (e[4] + wv[4]);
+        const R d2 = R(4) * cen[5] + (e[5] + wv[5]);
         const R tb_ = b0 + b2, tg = g0 + g2, td = d0 + d2;
         ymsg[buf][0][ry][ix] = tb_ + b1;
         ymsg[buf][1][ry][ix] = tb_ - b1;
@@ -578,7 +649,9 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         ymsg[buf][4][ry][ix] = td + d1_;
         ymsg[buf][5][ry][ix] = td - d1_;
     };
-    auto centres = [&](const XMsg<R> &xm, R (&cen)[6]) {
+    auto keep = [&](const XMsg<R> &xm, R (&xv)[10], R (&cen)[6]) {
+#pragma unroll
+        for (int v = 0; v < 10; v++) xv[v] = xm.v[v];
         cen[0] = xm.cA;
         cen[1] = xm.cC;
         cen[2] = xm.v[4];
@@ -587,15 +660,21 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         cen[5] = xm.v[9];
     };
 
+    // Iterations t = 0 .. n+1.  t = 0 and 1 are peeled (stages B, C not yet
+    // active); from t = 2 on every stage runs: the stage A / B work of
+    // t = n, n+1 concerns planes past the chunk and only touches registers
+    // and LDS buffers that are not read again.  Keeping the main loop free
+    // of branches around memory instructions keeps the memory counters
+    // exact, so queue loads stay in flight for several iterations.
     const int nit = n + 2;
 
-    if (wave < TY) {
-        // ================= interior columns =================
+    if (wave < TYP) {
+        // ================= interior rows =================
         const int ix = lane, iy = wave;
         const int x = x0 + ix, y = y0 + iy;
-        const int rx = ix + 1, ry = iy + 1;
+        const int ry = iy + 1;
         const bool inside = x < nx && y < ny;
-        const int col = colx(x) + nx * coly(y);
+        const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
         R om0 = 0, omx = 0, omy = 0;
         if (!PERIODIC) {
@@ -603,16 +682,23 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
             omx = omega[nxy + col];
             omy = omega[2 * nxy + col];
         }
-        const bool store = inside && fluid;
+        // Lanes that own no fluid cell store 0 to their (wall) column: in
+        // a walled box an outside lane's clamped column is a wall.  In a
+        // periodic box an outside lane is a copy of an inside lane and
+        // stores the identical value.
+        const bool store = PERIODIC ? true : (inside && fluid);
+        const unsigned cb_ = col * (unsigned)sizeof(S);
+        const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oU = mkbuf(out + N, fbytes),
+                                     oV = mkbuf(out + 2 * N, fbytes), oW = mkbuf(out + 3 * N, fbytes);
         // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
         S qp[QL], qx[QL], qy[QL], qz[QL];
-        auto qload = [&](int q, int s) {
+        auto qload = [&](int q, int s) __attribute__((always_inline)) {
             R jump;
-            const int gi = col + nxy * phys(q, jump);
-            qp[s] = Pi[gi];
-            qx[s] = Ui[gi];
-            qy[s] = Vi[gi];
-            qz[s] = Wi[gi];
+            const unsigned so = poff(phys(q, jump));
+            qp[s] = bload<S>(rP, cb_, so);
+            qx[s] = bload<S>(rU, cb_, so);
+            qy[s] = bload<S>(rV, cb_, so);
+            qz[s] = bload<S>(rW, cb_, so);
         };
 #pragma unroll
         for (int j = 2; j < QL - 1; j++) qload(zs - 2 - HF + j, j);
@@ -620,119 +706,138 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         mq[0] = msg_of(qp[HF + 1], qx[HF + 1], qy[HF + 1], qz[HF + 1], zs - 1, fluid);
         mq[1] = msg_of(qp[HF + 2], qx[HF + 2], qy[HF + 2], qz[HF + 2], zs, fluid);
         mq[2] = zero_msg<R>();
-        R cen[6] = {0, 0, 0, 0, 0, 0};
+        R xv[10], cen[6];
+#pragma unroll
+        for (int v = 0; v < 10; v++) xv[v] = 0;
+#pragma unroll
+        for (int v = 0; v < 6; v++) cen[v] = 0;
         R b0c = 0, g0c = 0, d0c = 0;
 
-        for (int ib = 0; ib < nit; ib += U) {
+        // one iteration; k = t mod 12 (a constant after unrolling)
+        auto iter = [&](int t, int k, bool doB, bool doC) __attribute__((always_inline)) {
+            const int a = zs + t;
+            qload(a + HF + LA, (k + QL - 1) % QL);
+            {
+                const int s1 = (k + 3 + HF) % QL;  // plane a+1
+                mq[(k + 2) % 3] = msg_of(qp[s1], qx[s1], qy[s1], qz[s1], a + 1, fluid);
+            }
+            if (doC) {  // stage C: plane c = a-2
+                const int c = a - 2;
+                const int cb = c & 1;
+                const int sc = (k + HF) % QL;
+                const R QPp = ymsg[cb][0][ry + 1][ix], QPm = ymsg[cb][1][ry - 1][ix];
+                const R QXp = ymsg[cb][2][ry + 1][ix], QXm = ymsg[cb][3][ry - 1][ix];
+                const R QZp = ymsg[cb][4][ry + 1][ix], QZm = ymsg[cb][5][ry - 1][ix];
+                constexpr R inv216 = R(1) / R(216);
+                const R ux = ST::ld_u(qx[sc], P.q), uy = ST::ld_u(qy[sc], P.q),
+                        uz = ST::ld_u(qz[sc], P.q);
+                // unnormalised predictor sums (x216)
+                const R Ps = R(4) * b0c + (QPp + QPm);
+                const R Xs = R(4) * g0c + (QXp + QXm);
+                const R Ys = QPp - QPm;
+                const R Zs = R(4) * d0c + (QZp + QZm);
+                // wall rule terms: own pressure times the wall weights
+                R w0 = 0, wx = 0, wy = 0;
+                if (!PERIODIC) {
+                    const R ps = fluid ? ST::ld_p(qp[sc], c, P.q) : R(0);
+                    w0 = ps * om0;
+                    wx = ps * omx;
+                    wy = ps * omy;
+                }
+                const int cy_ = HF + iy, cx_ = HF + ix;
+                const FdCell<R> xp1 = fd[cb][cy_][cx_ + D1], xm1 = fd[cb][cy_][cx_ - D1];
+                const FdCell<R> yp1 = fd[cb][cy_ + D1][cx_], ym1 = fd[cb][cy_ - D1][cx_];
+                const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
+                const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
+                auto U_ = [&](const S *qq, int sl) { return ST::ld_u(qq[sl], P.q); };
+                // corrector arm sums at distance d1 (and 2*d1)
+                const R s1x = xp1.x + xm1.x + yp1.x + ym1.x + U_(qx, zp1) + U_(qx, zm1);
+                const R s1y = xp1.y + xm1.y + yp1.y + ym1.y + U_(qy, zp1) + U_(qy, zm1);
+                const R s1z = xp1.z + xm1.z + yp1.z + ym1.z + U_(qz, zp1) + U_(qz, zm1);
+                // u' = tb (u*/216 + wall + g (c0 u + c1 S1 + c2 S2)) + (1 - tb) u
+                R nx_ = ux * P.ku + s1x * P.ks1;
+                R ny_ = uy * P.ku + s1y * P.ks1;
+                R nz_ = uz * P.ku + s1z * P.ks1;
+                if constexpr (FD2) {  // order-4 arms (blend != 0)
+                    const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
+                    const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
+                    nx_ += P.ks2 * (xp2.x + xm2.x + yp2.x + ym2.x + U_(qx, zp2) + U_(qx, zm2));
+                    ny_ += P.ks2 * (xp2.y + xm2.y + yp2.y + ym2.y + U_(qy, zp2) + U_(qy, zm2));
+                    nz_ += P.ks2 * (xp2.z + xm2.z + yp2.z + ym2.z + U_(qz, zp2) + U_(qz, zm2));
+                }
+                nx_ += P.tb * wx;
+                ny_ += P.tb * wy;
+                nx_ += Xs * P.kx;
+                ny_ += Ys * P.kx;
+                nz_ += Zs * P.kx;
+                const R np_ = (Ps * inv216 + w0) - P.gauge;
+                const unsigned so = poff(c);
+                const S z0 = S(0);
+                bstore(store ? ST::st_p(np_, c, P.q) : z0, oP, cb_, so);
+                bstore(store ? ST::st_u(nx_, P.q) : z0, oU, cb_, so);
+                bstore(store ? ST::st_u(ny_, P.q) : z0, oV, cb_, so);
+                bstore(store ? ST::st_u(nz_, P.q) : z0, oW, cb_, so);
+            }
+            if (doB) {  // stage B: plane a-1; corrector tile of a-1
+                xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
+                const int sb = (k + 1 + HF) % QL;
+                FdCell<R> cc;
+                cc.x = ST::ld_u(qx[sb], P.q);
+                cc.y = ST::ld_u(qy[sb], P.q);
+                cc.z = ST::ld_u(qz[sb], P.q);
+                fd[(a - 1) & 1][HF + iy][HF + ix] = cc;
+            }
+            // stage A: plane a
+            keep(zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]), xv, cen);
+            __syncthreads();
+        };
+        iter(0, 0, false, false);
+        iter(1, 1, true, false);
+        for (int ib = 2; ib < nit; ib += U) {
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 const int t = ib + k;
                 if (t >= nit) break;
-                const int a = zs + t;
-                qload(a + HF + LA, (k + QL - 1) % QL);
-                {
-                    const int s1 = (k + 3 + HF) % QL;  // plane a+1
-                    mq[(k + 2) % 3] = msg_of(qp[s1], qx[s1], qy[s1], qz[s1], a + 1, fluid);
-                }
-                if (t >= 2) {  // stage C: plane c = a-2
-                    const int c = a - 2;
-                    const int cb = c & 1;
-                    const int sc = (k + HF) % QL;
-                    const R QPp = ymsg[cb][0][ry + 1][ix], QPm = ymsg[cb][1][ry - 1][ix];
-                    const R QXp = ymsg[cb][2][ry + 1][ix], QXm = ymsg[cb][3][ry - 1][ix];
-                    const R QZp = ymsg[cb][4][ry + 1][ix], QZm = ymsg[cb][5][ry - 1][ix];
-                    constexpr R inv216 = R(1) / R(216);
-                    R Ps = (R(4) * b0c + (QPp + QPm)) * inv216;
-                    R Xs = (R(4) * g0c + (QXp + QXm)) * inv216;
-                    R Ys = (QPp - QPm) * inv216;
-                    R Zs = (R(4) * d0c + (QZp + QZm)) * inv216;
-                    const R ux = ST::ld_u(qx[sc], P.q), uy = ST::ld_u(qy[sc], P.q),
-                            uz = ST::ld_u(qz[sc], P.q);
-                    if (!PERIODIC) {
-                        const R ps = fluid ? ST::ld_p(qp[sc], c, P.q) : R(0);
-                        Ps += ps * om0;
-                        Xs += ps * omx;
-                        Ys += ps * omy;
-                    }
-                    const int cy_ = HF + iy, cx_ = HF + ix;
-                    const FdCell<R> xp1 = fd[cb][cy_][cx_ + D1], xm1 = fd[cb][cy_][cx_ - D1];
-                    const FdCell<R> yp1 = fd[cb][cy_ + D1][cx_], ym1 = fd[cb][cy_ - D1][cx_];
-                    const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
-                    const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
-                    const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
-                    const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
-                    auto U_ = [&](const S *qq, int sl) { return ST::ld_u(qq[sl], P.q); };
-                    const R Lx = P.c0 * ux +
-                                 P.c1 * (xp1.x + xm1.x + yp1.x + ym1.x + U_(qx, zp1) + U_(qx, zm1)) +
-                                 P.c2 * (xp2.x + xm2.x + yp2.x + ym2.x + U_(qx, zp2) + U_(qx, zm2));
-                    const R Ly = P.c0 * uy +
-                                 P.c1 * (xp1.y + xm1.y + yp1.y + ym1.y + U_(qy, zp1) + U_(qy, zm1)) +
-                                 P.c2 * (xp2.y + xm2.y + yp2.y + ym2.y + U_(qy, zp2) + U_(qy, zm2));
-                    const R Lz = P.c0 * uz +
-                                 P.c1 * (xp1.z + xm1.z + yp1.z + ym1.z + U_(qz, zp1) + U_(qz, zm1)) +
-                                 P.c2 * (xp2.z + xm2.z + yp2.z + ym2.z + U_(qz, zp2) + U_(qz, zm2));
-                    if (store) {
-                        const int go = col + nxy * c;
-                        out[go] = ST::st_p(Ps - P.gauge, c, P.q);
-                        out[N + go] = ST::st_u((Xs + P.g * Lx) * P.tb + ux * P.cc, P.q);
-                        out[2 * N + go] = ST::st_u((Ys + P.g * Ly) * P.tb + uy * P.cc, P.q);
-                        out[3 * N + go] = ST::st_u((Zs + P.g * Lz) * P.tb + uz * P.cc, P.q);
-                    }
-                }
-                if (t >= 1 && t <= n) {  // stage B: plane a-1; corrector tile of a-1
-                    xpass((a - 1) & 1, ry, rx, ix, cen, b0c, g0c, d0c);
-                    const int sb = (k + 1 + HF) % QL;
-                    FdCell<R> cc;
-                    cc.x = ST::ld_u(qx[sb], P.q);
-                    cc.y = ST::ld_u(qy[sb], P.q);
-                    cc.z = ST::ld_u(qz[sb], P.q);
-                    fd[(a - 1) & 1][HF + iy][HF + ix] = cc;
-                }
-                if (t < n) {  // stage A: plane a
-                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
-                    store_x(a & 1, ry, rx, xm);
-                    centres(xm, cen);
-                }
-                __syncthreads();
+                iter(t, (k + 2) % U, true, true);
             }
         }
-    } else if (wave < TY + 2) {
-        // ================= halo rows y0-1 and y0+TY =================
-        const int hr = wave - TY;
+    } else if (wave < TYP + 2) {
+        // ================= halo rows y0-1 and y0+TYP =================
+        const int hr = wave - TYP;
         const int ix = lane;
         const int x = x0 + ix;
-        const int y = hr ? y0 + TY : y0 - 1;
-        const int rx = ix + 1, ry = hr ? TY + 1 : 0;
+        const int y = hr ? y0 + TYP : y0 - 1;
+        const int ry = hr ? TYP + 1 : 0;
         const bool inside = x < nx && y >= 0 && y < ny;
-        const int col = colx(x) + nx * coly(y);
+        const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        const unsigned cb_ = col * (unsigned)sizeof(S);
         // own column prefetch: plane m in slot (m - zs + 1) mod PD
         S hp[PD], hx[PD], hy[PD], hz[PD];
-        auto hload = [&](int m, int s) {
+        auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
-            const int gi = col + nxy * phys(m, jump);
-            hp[s] = Pi[gi];
-            hx[s] = Ui[gi];
-            hy[s] = Vi[gi];
-            hz[s] = Wi[gi];
+            const unsigned so = poff(phys(m, jump));
+            hp[s] = bload<S>(rP, cb_, so);
+            hx[s] = bload<S>(rU, cb_, so);
+            hy[s] = bload<S>(rV, cb_, so);
+            hz[s] = bload<S>(rW, cb_, so);
         };
         // corrector y arms: HF rows of this side, column ix; plane q in
         // slot (q - zs + 1) mod AD
-        int acol[HF];
+        unsigned acol[HF];
 #pragma unroll
         for (int r = 0; r < HF; r++) {
-            const int fy = hr ? TY + HF + r : r;
-            acol[r] = colx(x0 + ix) + nx * coly(y0 - HF + fy);
+            const int fy = hr ? TYP + HF + r : r;
+            acol[r] = (colx(x0 + ix) + nx * coly(y0 - HF + fy)) * (unsigned)sizeof(S);
         }
         S ar[AD][HF][3];
-        auto aload = [&](int q, int s) {
-            const int zp = wrapi(q, nz);
+        auto aload = [&](int q, int s) __attribute__((always_inline)) {
+            const unsigned so = poff(wrapi(q, nz));
 #pragma unroll
             for (int r = 0; r < HF; r++) {
-                const int gi = acol[r] + nxy * zp;
-                ar[s][r][0] = Ui[gi];
-                ar[s][r][1] = Vi[gi];
-                ar[s][r][2] = Wi[gi];
+                ar[s][r][0] = bload<S>(rU, acol[r], so);
+                ar[s][r][1] = bload<S>(rV, acol[r], so);
+                ar[s][r][2] = bload<S>(rW, acol[r], so);
             }
         };
 #pragma unroll
@@ -745,76 +850,91 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
-        R cen[6] = {0, 0, 0, 0, 0, 0};
+        R xv[10], cen[6];
+#pragma unroll
+        for (int v = 0; v < 10; v++) xv[v] = 0;
+#pragma unroll
+        for (int v = 0; v < 6; v++) cen[v] = 0;
         R b0c, g0c, d0c;
 
-        for (int ib = 0; ib < nit; ib += U) {
+        auto iter = [&](int t, int k, bool doB) __attribute__((always_inline)) {
+            const int a = zs + t;
+            {
+                const int s1 = (k + 2) % PD;  // plane a+1
+                mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                hload(a + 1 + PD, s1);
+            }
+            if (doB) {
+                xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
+                const int sa = k % AD;  // plane a-1
+#pragma unroll
+                for (int r = 0; r < HF; r++) {
+                    const int fy = hr ? TYP + HF + r : r;
+                    FdCell<R> cc;
+                    cc.x = ST::ld_u(ar[sa][r][0], P.q);
+                    cc.y = ST::ld_u(ar[sa][r][1], P.q);
+                    cc.z = ST::ld_u(ar[sa][r][2], P.q);
+                    fd[(a - 1) & 1][fy][HF + ix] = cc;
+                }
+            }
+            aload(a + 2, k % AD);
+            keep(zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]), xv, cen);
+            __syncthreads();
+        };
+        iter(0, 0, false);
+        iter(1, 1, true);
+        for (int ib = 2; ib < nit; ib += U) {
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 const int t = ib + k;
                 if (t >= nit) break;
-                const int a = zs + t;
-                {
-                    const int s1 = (k + 2) % PD;  // plane a+1
-                    mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
-                    hload(a + 1 + PD, s1);
-                }
-                if (t >= 1 && t <= n) {
-                    xpass((a - 1) & 1, ry, rx, ix, cen, b0c, g0c, d0c);
-                    const int sa = k % AD;  // plane a-1
-#pragma unroll
-                    for (int r = 0; r < HF; r++) {
-                        const int fy = hr ? TY + HF + r : r;
-                        FdCell<R> cc;
-                        cc.x = ST::ld_u(ar[sa][r][0], P.q);
-                        cc.y = ST::ld_u(ar[sa][r][1], P.q);
-                        cc.z = ST::ld_u(ar[sa][r][2], P.q);
-                        fd[(a - 1) & 1][fy][HF + ix] = cc;
-                    }
-                }
-                aload(a + 2, k % AD);
-                if (t < n) {
-                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
-                    store_x(a & 1, ry, rx, xm);
-                    centres(xm, cen);
-                }
-                __syncthreads();
+                iter(t, (k + 2) % U, true);
             }
         }
     } else {
-        // ======== halo columns x0-1, x0+TX (lanes < 2*RY) and x arms ========
-        const int sl = lane < NS ? lane : NS - 1;
-        const int side = sl / RY, r = sl - side * RY;
+        // ===== halo columns x0-1, x0+64 (lanes < 2*RYP) and the x arms =====
+        const int sl = lane < 2 * RYP ? lane : 2 * RYP - 1;
+        const int side = sl / RYP, r = sl - side * RYP;
         const int x = side ? x0 + TX : x0 - 1;
         const int y = y0 - 1 + r;
-        const int rx = side ? TX + 1 : 0, ry = r;
+        const int ry = r;
         const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
-        const int col = colx(x) + nx * coly(y);
+        const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
-        const bool writer = lane < NS;
+        const bool writer = lane < 2 * RYP;
+        const unsigned cb_ = col * (unsigned)sizeof(S);
         S hp[PD], hx[PD], hy[PD], hz[PD];
-        auto hload = [&](int m, int s) {
+        auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
-            const int gi = col + nxy * phys(m, jump);
-            hp[s] = Pi[gi];
-            hx[s] = Ui[gi];
-            hy[s] = Vi[gi];
-            hz[s] = Wi[gi];
+            const unsigned so = poff(phys(m, jump));
+            hp[s] = bload<S>(rP, cb_, so);
+            hx[s] = bload<S>(rU, cb_, so);
+            hy[s] = bload<S>(rV, cb_, so);
+            hz[s] = bload<S>(rW, cb_, so);
         };
-        // x arms: 2*HF*TY cells, one per lane
-        const int na = 2 * HF * TY;
-        const int al = lane < na ? lane : na - 1;
-        const bool awriter = lane < na;
-        const int br = al / (2 * HF), bc = al - br * (2 * HF);
-        const int afy = HF + br;
-        const int afx = bc < HF ? bc : TX + bc;
-        const int acol = colx(x0 - HF + afx) + nx * coly(y0 - HF + afy);
-        S ar[AD][3];
-        auto aload = [&](int q, int s) {
-            const int gi = acol + nxy * wrapi(q, nz);
-            ar[s][0] = Ui[gi];
-            ar[s][1] = Vi[gi];
-            ar[s][2] = Wi[gi];
+        // x arms: NXA cells, XPL per lane
+        int afy[XPL], afx[XPL];
+        bool aw[XPL];
+        unsigned acol[XPL];
+#pragma unroll
+        for (int j = 0; j < XPL; j++) {
+            const int cell = lane + 64 * j;
+            aw[j] = cell < NXA;
+            const int cl = aw[j] ? cell : NXA - 1;
+            const int br = cl / (2 * HF), bc = cl - br * (2 * HF);
+            afy[j] = HF + br;
+            afx[j] = bc < HF ? bc : TX + bc;
+            acol[j] = (colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j])) * (unsigned)sizeof(S);
+        }
+        S ar[AD][XPL][3];
+        auto aload = [&](int q, int s) __attribute__((always_inline)) {
+            const unsigned so = poff(wrapi(q, nz));
+#pragma unroll
+            for (int j = 0; j < XPL; j++) {
+                ar[s][j][0] = bload<S>(rU, acol[j], so);
+                ar[s][j][1] = bload<S>(rV, acol[j], so);
+                ar[s][j][2] = bload<S>(rW, acol[j], so);
+            }
         };
 #pragma unroll
         for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
@@ -827,31 +947,50 @@ __global__ __launch_bounds__(NT) void k_step_pipe(
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
 
-        for (int ib = 0; ib < nit; ib += U) {
+        auto iter = [&](int t, int k, bool doB) __attribute__((always_inline)) {
+            const int a = zs + t;
+            {
+                const int s1 = (k + 2) % PD;
+                mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                hload(a + 1 + PD, s1);
+            }
+            if (doB) {
+                const int sa = k % AD;
+#pragma unroll
+                for (int j = 0; j < XPL; j++) {
+                    if (!aw[j]) continue;
+                    FdCell<R> cc;
+                    cc.x = ST::ld_u(ar[sa][j][0], P.q);
+                    cc.y = ST::ld_u(ar[sa][j][1], P.q);
+                    cc.z = ST::ld_u(ar[sa][j][2], P.q);
+                    fd[(a - 1) & 1][afy[j]][afx[j]] = cc;
+                }
+            }
+            aload(a + 2, k % AD);
+            if (writer) {
+                // edge block of this column: the values its neighbour lane
+                // reads (left column -> XL, right column -> XR); two
+                // branches with static indexes (a select between array
+                // elements would become a dynamic index)
+                const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
+                if (side) {
+#pragma unroll
+                    for (int j = 0; j < 6; j++) edge[a & 1][ry][1][j] = xm.v[XR[j]];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 6; j++) edge[a & 1][ry][0][j] = xm.v[XL[j]];
+                }
+            }
+            __syncthreads();
+        };
+        iter(0, 0, false);
+        iter(1, 1, true);
+        for (int ib = 2; ib < nit; ib += U) {
 #pragma unroll
             for (int k = 0; k < U; k++) {
                 const int t = ib + k;
                 if (t >= nit) break;
-                const int a = zs + t;
-                {
-                    const int s1 = (k + 2) % PD;
-                    mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
-                    hload(a + 1 + PD, s1);
-                }
-                if (t >= 1 && t <= n && awriter) {
-                    const int sa = k % AD;
-                    FdCell<R> cc;
-                    cc.x = ST::ld_u(ar[sa][0], P.q);
-                    cc.y = ST::ld_u(ar[sa][1], P.q);
-                    cc.z = ST::ld_u(ar[sa][2], P.q);
-                    fd[(a - 1) & 1][afy][afx] = cc;
-                }
-                aload(a + 2, k % AD);
-                if (t < n && writer) {
-                    const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
-                    store_x(a & 1, ry, rx, xm);
-                }
-                __syncthreads();
+                iter(t, (k + 2) % U, true);
             }
         }
     }
@@ -871,9 +1010,14 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
     if constexpr (MODE == FSLBM_F64)
         k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
-    else
-        k_step_pipe<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
-            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    else if (P.c2 != R(0))
+        k_step_pipe<MODE, PERIODIC, D1, true, pipe_rows(MODE)>
+            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
+                (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    else  // blend = 0: the order-4 arms have zero weight
+        k_step_pipe<MODE, PERIODIC, D1, false, pipe_rows(MODE)>
+            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
+                (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     return hipGetLastError();
 }
 
@@ -888,9 +1032,10 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
     P.ny = g.ny;
     P.nz = g.nz;
     P.nxy = g.nx * g.ny;
+    const int rows = MODE == FSLBM_F64 ? TY : pipe_rows(MODE);  // tile rows of the kernel
     P.ntx = (g.nx + TX - 1) / TX;
-    P.nty = (g.ny + TY - 1) / TY;
-    P.chunk = aux.chunk > 0 ? aux.chunk : default_chunk(g);
+    P.nty = (g.ny + rows - 1) / rows;
+    P.chunk = aux.chunk > 0 ? aux.chunk : default_chunk(g, MODE);
     if (P.chunk > g.nz) P.chunk = g.nz;
     P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
     P.nwork = P.ntx * P.nty * P.nchunks;
@@ -903,6 +1048,10 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
     P.g = k.dtddy * k.m;
     P.tb = k.c;
     P.cc = k.cc;
+    P.ku = P.tb * P.g * P.c0 + P.cc;
+    P.ks1 = P.tb * P.g * P.c1;
+    P.ks2 = P.tb * P.g * P.c2;
+    P.kx = P.tb / R(216);
     P.q = a.q;
     const bool per = g.periodic != 0;
     if (a.d1 == 1)
@@ -914,11 +1063,12 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
 
 }  // namespace
 
-int default_chunk(const Geo &g)
+int default_chunk(const Geo &g, int precision)
 {
-    // Enough work items to give every CU ~4 tiles, chunks of >= 16 planes.
-    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY);
-    long long c = (g.nz * tiles + 1023) / 1024;
+    // About three work items per CU (256 CUs), chunks of >= 16 planes.
+    const int rows = precision == FSLBM_F64 ? TY : pipe_rows(precision);
+    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
+    long long c = (g.nz * tiles + 767) / 768;
     if (c < 16) c = 16;
     if (c > g.nz) c = g.nz;
     return (int)c;

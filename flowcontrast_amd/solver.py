@@ -106,6 +106,13 @@ class Solver:
     def tile_chunk(self) -> int:
         return self._chk(lib().fslbm_get_tile_chunk(self._h))
 
+    def autotune(self, trial_steps: int = 5):
+        """Pick the fastest tiled schedule; returns (chunk, ms per step).
+        The state is not advanced."""
+        best, ms = C.c_int(), C.c_double()
+        self._chk(lib().fslbm_autotune(self._h, int(trial_steps), C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
     # -- state ----------------------------------------------------------------
     def set_state(self, p, ux, uy, uz) -> None:
         arrs = [np.ascontiguousarray(a).reshape(-1) for a in (p, ux, uy, uz)]

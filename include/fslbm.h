@@ -102,6 +102,14 @@ int fslbm_get_kernel(fslbm_t ctx);
 int fslbm_set_tile_chunk(fslbm_t ctx, int chunk);
 int fslbm_get_tile_chunk(fslbm_t ctx);
 
+/* Autotuner of the tiled kernel's schedule: times trial_steps launches for
+ * each candidate chunk (HIP events on the context's stream), keeps the
+ * fastest and reports it (*best_chunk, *best_ms per step; either may be
+ * NULL).  The solver state is not advanced: trial launches read the
+ * current state and write the spare buffer.  No-op for the plain kernel. */
+int fslbm_autotune(fslbm_t ctx, int trial_steps, int *best_chunk,
+                   double *best_ms);
+
 /* Upload / download the whole state as four SoA arrays of nx*ny*nz values
  * (p, ux, uy, uz).  The uploaded state is taken as the state after a
  * bcond: wall cells are forced to 0 (FSLBM.cu:290).  Replaces the restart

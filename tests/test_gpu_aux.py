@@ -219,3 +219,28 @@ def test_full_size_pipe_conserves_subspace(variant):
     assert np.abs(got[0] - p).max() < 2e-8
     # z-invariance of the axial velocity is kept up to roundoff
     assert np.abs(got[3] - got[3][:1]).max() < 1e-6 * 0.05
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner times trial launches without advancing the state, and
+    the chunk it picks changes only the schedule: results are bitwise the
+    same as with the default schedule."""
+    nx, ny, nz, R = 70, 60, 96, 26.0
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    st = inputs.poiseuille_state(mask, R, 0.04, 1e-3, 0.05, 9)
+    prm = RunParams(tblend=0.8, pin=1e-3, nu=0.01, d1=2, delta=1, blend=0.5)
+    outs = []
+    for tune in (False, True):
+        with Solver(mask.shape, prm, mask=mask, precision="f32") as s:
+            s.set_state(*[a.astype(np.float32) for a in st])
+            if tune:
+                before = s.get_state(np.float32)
+                chunk, ms = s.autotune(2)
+                assert chunk >= 16 and ms > 0
+                after = s.get_state(np.float32)
+                for a, b in zip(before, after):
+                    assert np.array_equal(a, b)
+            s.step(9)
+            outs.append(s.get_state(np.float32))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
