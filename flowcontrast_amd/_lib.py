@@ -30,7 +30,13 @@ EXPORTS = [
     "fslbm_moments_read", "fslbm_sync", "fslbm_time_steps",
     "fslbm_device_bytes", "fslbm_pipe_m_mask", "fslbm_circle_mask",
     "fslbm_init_poiseuille", "fslbm_init_taylor_green", "fslbm_fold_mask",
+    "fslbm_batch_create", "fslbm_batch_destroy", "fslbm_batch_last_error",
+    "fslbm_batch_domain_bytes", "fslbm_batch_add", "fslbm_batch_remove",
+    "fslbm_batch_step", "fslbm_batch_get_state", "fslbm_batch_query",
+    "fslbm_batch_use_graph", "fslbm_batch_set_chunk", "fslbm_batch_autotune",
+    "fslbm_batch_time_steps", "fslbm_batch_sync",
 ]
+EBUSY = -6
 
 
 class Params(C.Structure):
@@ -103,6 +109,21 @@ def lib() -> C.CDLL:
         "fslbm_init_poiseuille": (i, [i, i, i, _u8, d, d, d, d, C.c_uint64, _dp, _dp, _dp, _dp]),
         "fslbm_init_taylor_green": (i, [i, i, i, d, d, _dp, _dp, _dp, _dp]),
         "fslbm_fold_mask": (i, [i, i, i, i, _u8, _u8]),
+        "fslbm_batch_create": (i, [i, i, C.c_size_t, C.POINTER(vp)]),
+        "fslbm_batch_destroy": (None, [vp]),
+        "fslbm_batch_last_error": (C.c_char_p, [vp]),
+        "fslbm_batch_domain_bytes": (C.c_longlong, [i, C.c_void_p, i, i, i]),
+        "fslbm_batch_add": (i, [vp, C.POINTER(Params), C.c_void_p, i, i, i, _dp, _dp, _dp, _dp]),
+        "fslbm_batch_remove": (i, [vp, i]),
+        "fslbm_batch_step": (i, [vp, i]),
+        "fslbm_batch_get_state": (i, [vp, i, _dp, _dp, _dp, _dp]),
+        "fslbm_batch_query": (i, [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), ip,
+                                  C.POINTER(C.c_longlong)]),
+        "fslbm_batch_use_graph": (i, [vp, i]),
+        "fslbm_batch_set_chunk": (i, [vp, i]),
+        "fslbm_batch_autotune": (i, [vp, i, ip, C.POINTER(d)]),
+        "fslbm_batch_time_steps": (i, [vp, i, C.POINTER(d)]),
+        "fslbm_batch_sync": (i, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -113,7 +134,9 @@ def lib() -> C.CDLL:
 
 
 class FslbmError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc: int, ctx=None) -> int:
@@ -121,5 +144,13 @@ def check(rc: int, ctx=None) -> int:
         L = lib()
         msg = L.fslbm_last_error(ctx).decode() if ctx is not None else \
             L.fslbm_last_error(None).decode()
-        raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}")
+        raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}", rc)
+    return rc
+
+
+def check_batch(rc: int, b=None) -> int:
+    if rc < 0:
+        L = lib()
+        msg = L.fslbm_batch_last_error(b).decode()
+        raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}", rc)
     return rc

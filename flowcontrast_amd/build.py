@@ -36,7 +36,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 LIB_SOURCES = [
     "fslbm_kernels.hip",
     "fslbm_tiled.hip",
-    "fslbm_batch.hip",
+    "fslbm_batch.cpp",
     "fslbm_api.cpp",
     "fslbm_inputs.cpp",
 ]

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "fslbm_codec.h"
 #include "fslbm_internal.h"
 #include "fslbm_store.h"
 
@@ -98,103 +99,13 @@ int check_params(fslbm_ctx *c, const fslbm_params *p)
     return FSLBM_OK;
 }
 
-template <typename T>
-void encode(const fslbm_ctx *c, const T *const src[4], void *dst)
+HostMask host_mask(const fslbm_ctx *c)
 {
-    const long long N = c->g.N;
-    const int nxy = c->g.nx * c->g.ny;
-    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
-    auto run = [&](auto tag) {
-        using ST = decltype(tag);
-        using S = typename ST::S;
-        S *d = (S *)dst;
-        for (long long i = 0; i < N; i++) {
-            const bool wall = m && m[i] != 0;
-            const int z = (int)(i / nxy);
-            d[i] = wall ? S(0) : ST::st_p((typename ST::R)src[0][i], z, c->q);
-            for (int f = 1; f < 4; f++)
-                d[f * N + i] = wall ? S(0) : ST::st_u((typename ST::R)src[f][i], c->q);
-        }
-    };
-    switch (c->precision) {
-    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
-    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
-    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
-    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
-    }
-}
-
-template <typename T>
-void decode(const fslbm_ctx *c, const void *src, T *const dst[4])
-{
-    const long long N = c->g.N;
-    const int nxy = c->g.nx * c->g.ny;
-    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
-    auto run = [&](auto tag) {
-        using ST = decltype(tag);
-        using S = typename ST::S;
-        const S *s = (const S *)src;
-        for (long long i = 0; i < N; i++) {
-            const bool wall = m && m[i] != 0;
-            const int z = (int)(i / nxy);
-            dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, c->q);
-            for (int f = 1; f < 4; f++)
-                dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], c->q);
-        }
-    };
-    switch (c->precision) {
-    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
-    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
-    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
-    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
-    }
-}
-
-// Smallest power of two >= v (v > 0), as a float.
-float pow2_at_least(double v)
-{
-    int e;
-    double m = std::frexp(v, &e);  // v = m * 2^e, m in [0.5, 1)
-    if (m == 0.5) e -= 1;
-    return std::ldexp(1.0f, e);
-}
-
-// Choose the shifted-storage reference and scales from the state: the
-// pressure reference is the mean plus the mean axial gradient -pin/nz;
-// scales map the largest magnitude to [512, 1024) (exact powers of two).
-template <typename T>
-void choose_quant(fslbm_ctx *c, const T *const src[4])
-{
-    const long long N = c->g.N;
-    const int nxy = c->g.nx * c->g.ny;
-    const uint8_t *m = c->g.periodic ? nullptr : c->hmask.data();
-    const double slope = c->g.periodic ? 0.0 : -c->prm.pin / c->g.nz;
-    double sum = 0;
-    long long n = 0;
-    for (long long i = 0; i < N; i++) {
-        if (m && m[i] != 0) continue;
-        sum += (double)src[0][i] - slope * (double)(i / nxy);
-        n++;
-    }
-    const double base = n ? sum / (double)n : 0.0;
-    double pmax = 0, umax = 0;
-    for (long long i = 0; i < N; i++) {
-        if (m && m[i] != 0) continue;
-        double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
-        pmax = std::max(pmax, std::fabs(dp));
-        for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
-    }
-    // Headroom for growth: at least 1e-3 for velocity, 1e-4 for pressure.
-    umax = std::max(umax, 1e-3);
-    pmax = std::max(pmax, std::max(1e-4, std::fabs(c->prm.pin)));
-    const float su = pow2_at_least(umax) / 1024.0f;
-    const float sp = pow2_at_least(pmax) / 1024.0f;
-    c->q.su = su;
-    c->q.inv_su = 1.0f / su;
-    c->q.sp = sp;
-    c->q.inv_sp = 1.0f / sp;
-    c->q.pbase = (float)base;
-    c->q.pslope = (float)slope;
+    HostMask hm;
+    hm.m = c->g.periodic ? nullptr : c->hmask.data();
+    hm.nxy = (long long)c->g.nx * c->g.ny;
+    hm.plane = false;
+    return hm;
 }
 
 template <typename T>
@@ -204,12 +115,13 @@ int set_state_impl(fslbm_ctx *c, const T *p, const T *ux, const T *uy,
     if (!c) return FSLBM_EINVAL;
     if (!p || !ux || !uy || !uz) return fail(c, FSLBM_EINVAL, "NULL state array");
     const T *src[4] = {p, ux, uy, uz};
+    const HostMask hm = host_mask(c);
     if (c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S)
-        choose_quant(c, src);
+        choose_quant(c->g, hm, c->prm.pin, src, c->q);
     const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
     void *host = nullptr;
     HIPCHK(c, hipHostMalloc(&host, bytes, hipHostMallocDefault));
-    encode(c, src, host);
+    encode_state(c->g, hm, c->precision, c->q, src, host);
     hipError_t e = hipMemcpyAsync(c->state[c->cur], host, bytes,
                                   hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
@@ -236,7 +148,7 @@ int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) {
         T *dst[4] = {p, ux, uy, uz};
-        decode(c, host, dst);
+        decode_state(c->g, host_mask(c), c->precision, c->q, host, dst);
     }
     (void)hipHostFree(host);
     if (e != hipSuccess)
@@ -253,41 +165,6 @@ bool use_tiled(const fslbm_ctx *c)
 {
     if (c->kernel == FSLBM_KERNEL_PLAIN) return false;
     return tiled_ok(c, c->prm.d1);
-}
-
-// Wall weights of a z-invariant mask, per fluid column (x, y):
-// omega0 = sum of W_e, omegax = sum of W_e cx, omegay = sum of W_e cy over
-// the neighbours x+e that are walls (or outside the box).  The z sums of
-// w(cz) are 1 and of w(cz) cz are 0, so omega_z = 0.  These restore the
-// reference's wall rule (own pressure in place of a wall neighbour's,
-// FSLBM.cu:137-143) in the separable predictor.
-template <typename T>
-std::vector<T> wall_weights(int nx, int ny, const uint8_t *m2)
-{
-    const double w[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
-    const size_t nxy = (size_t)nx * ny;
-    std::vector<T> om(3 * nxy, T(0));
-    for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++) {
-            const size_t c = x + (size_t)nx * y;
-            if (m2[c] != 0) continue;
-            double o0 = 0, ox = 0, oy = 0;
-            for (int cy = -1; cy <= 1; cy++)
-                for (int cx = -1; cx <= 1; cx++) {
-                    const int xx = x + cx, yy = y + cy;
-                    const bool wall = xx < 0 || yy < 0 || xx >= nx || yy >= ny ||
-                                      m2[xx + (size_t)nx * yy] != 0;
-                    if (!wall) continue;
-                    const double W = w[cx + 1] * w[cy + 1];
-                    o0 += W;
-                    ox += W * cx;
-                    oy += W * cy;
-                }
-            om[c] = (T)o0;
-            om[nxy + c] = (T)ox;
-            om[2 * nxy + c] = (T)oy;
-        }
-    return om;
 }
 
 int launch_one_step(fslbm_ctx *c, double gauge)
@@ -335,6 +212,7 @@ const char *fslbm_error_string(int code)
     case FSLBM_EHIP: return "HIP runtime error";
     case FSLBM_ENODEV: return "no usable gfx950 device";
     case FSLBM_ESTATE: return "invalid state for this call";
+    case FSLBM_EBUSY: return "HBM budget full";
     }
     return "unknown error";
 }

@@ -1,0 +1,539 @@
+// Batched domains: many independent FSLBM domains (pipes, periodic
+// boxes) advanced together, one persistent launch per kernel class and
+// step, over a packed table of (domain, work item).  Domains are admitted
+// against an HBM budget and removed when done; the device state of every
+// admitted domain stays resident.  The launch sequence of a step can be
+// captured in a hipGraph and replayed.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fslbm_codec.h"
+#include "fslbm_internal.h"
+
+using namespace fslbm;
+
+namespace {
+
+struct Domain {
+    bool active = false;
+    Geo g{};
+    fslbm_params prm{};
+    Quant q{1.f, 1.f, 1.f, 1.f, 0.f, 0.f};
+    void *buf[2] = {nullptr, nullptr};
+    uint8_t *mask2d = nullptr;
+    void *omega = nullptr;
+    std::vector<uint8_t> hmask2d;
+    size_t bytes = 0;
+    int parity = 0;  // buffer holding the current state
+    long steps = 0;
+};
+
+constexpr int NCLASS = 4;  // (periodic, order-4 arms)
+
+int class_of(const Domain &d) { return (d.g.periodic ? 2 : 0) + (d.prm.blend != 0.0 ? 1 : 0); }
+
+}  // namespace
+
+struct fslbm_batch {
+    int precision = FSLBM_F32;
+    int d1 = 2;
+    size_t budget = 0, used = 0;
+    std::vector<Domain> doms;
+    void *d_doms = nullptr;
+    size_t d_doms_cap = 0;
+    int2 *d_work[NCLASS] = {nullptr, nullptr, nullptr, nullptr};
+    size_t work_cap[NCLASS] = {0, 0, 0, 0};
+    int nwork[NCLASS] = {0, 0, 0, 0};
+    bool dirty = true;
+    int flip = 0;   // steps since the descriptors were written, mod 2
+    int chunk = 0;  // schedule (0: default per domain)
+    int grid = 256; // persistent workgroups (one per CU)
+    bool use_graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps from flip 0 / 1
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int bfail(fslbm_batch *b, int code, const char *fmt, ...)
+{
+    if (b) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        b->err = buf;
+    }
+    return code;
+}
+
+#define BCHK(b, expr)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return bfail((b), FSLBM_EHIP, "%s failed: %s", #expr,              \
+                         hipGetErrorString(e_));                               \
+    } while (0)
+
+thread_local std::string g_batch_err;
+
+StepArgs step_args(const Domain &d)
+{
+    StepArgs a;
+    a.tblend = d.prm.tblend;
+    a.pin = d.prm.pin;
+    a.nu = d.prm.nu;
+    a.blend = d.prm.blend;
+    a.d1 = d.prm.d1;
+    a.gauge = 0.0;
+    a.q = d.q;
+    return a;
+}
+
+void drop_graphs(fslbm_batch *b)
+{
+    for (auto &g : b->gexec)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// Rewrite the descriptors and work tables (after add/remove/retune).
+int rebuild(fslbm_batch *b, int chunk_override = -1)
+{
+    const size_t db = batch_dom_bytes(b->precision);
+    const size_t nd = b->doms.size();
+    std::vector<unsigned char> host(std::max<size_t>(nd, 1) * db, 0);
+    std::vector<int2> work[NCLASS];
+    for (size_t i = 0; i < nd; i++) {
+        Domain &d = b->doms[i];
+        if (!d.active) continue;
+        // the current parity of each domain is folded into its descriptor
+        d.parity ^= b->flip;
+        const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
+        const int nw = batch_fill_dom(host.data() + i * db, b->precision, d.g, step_args(d), ch,
+                                      d.buf[0], d.buf[1], d.mask2d, d.omega, d.parity);
+        if (nw < 0) return bfail(b, FSLBM_EINVAL, "domain %zu: unsupported precision", i);
+        auto &wl = work[class_of(d)];
+        for (int w = 0; w < nw; w++) wl.push_back(make_int2((int)i, w));
+    }
+    b->flip = 0;
+    if (host.size() > b->d_doms_cap) {
+        if (b->d_doms) (void)hipFree(b->d_doms);
+        b->d_doms = nullptr;
+        b->d_doms_cap = 0;
+        BCHK(b, hipMalloc(&b->d_doms, host.size()));
+        b->d_doms_cap = host.size();
+    }
+    BCHK(b, hipMemcpyAsync(b->d_doms, host.data(), host.size(), hipMemcpyHostToDevice, b->stream));
+    for (int c = 0; c < NCLASS; c++) {
+        const size_t need = std::max<size_t>(work[c].size(), 1) * sizeof(int2);
+        if (need > b->work_cap[c]) {
+            if (b->d_work[c]) (void)hipFree(b->d_work[c]);
+            b->d_work[c] = nullptr;
+            b->work_cap[c] = 0;
+            BCHK(b, hipMalloc((void **)&b->d_work[c], need));
+            b->work_cap[c] = need;
+        }
+        if (!work[c].empty())
+            BCHK(b, hipMemcpyAsync(b->d_work[c], work[c].data(), work[c].size() * sizeof(int2),
+                                   hipMemcpyHostToDevice, b->stream));
+        b->nwork[c] = (int)work[c].size();
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    drop_graphs(b);
+    b->dirty = false;
+    return FSLBM_OK;
+}
+
+// Launch one step of every class with the given flip (no bookkeeping).
+hipError_t launch_step_all(fslbm_batch *b, int flip)
+{
+    for (int c = 0; c < NCLASS; c++) {
+        if (b->nwork[c] == 0) continue;
+        BatchLaunch L;
+        L.doms = b->d_doms;
+        L.work = b->d_work[c];
+        L.nwork = b->nwork[c];
+        L.flip = flip;
+        L.grid = std::min(b->grid, 8 * ((b->nwork[c] + 7) / 8));
+        L.periodic = (c & 2) != 0;
+        L.fd2 = (c & 1) != 0;
+        L.d1 = b->d1;
+        L.precision = b->precision;
+        L.stream = b->stream;
+        const hipError_t e = launch_step_batch(L);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int capture_graphs(fslbm_batch *b)
+{
+    for (int start = 0; start < 2; start++) {
+        hipGraph_t g = nullptr;
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = launch_step_all(b, start);
+        hipError_t e2 = launch_step_all(b, start ^ 1);
+        hipError_t e3 = hipStreamEndCapture(b->stream, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return bfail(b, FSLBM_EHIP, "graph capture failed: %s",
+                         hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+        }
+        hipError_t e4 = hipGraphInstantiate(&b->gexec[start], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e4 != hipSuccess)
+            return bfail(b, FSLBM_EHIP, "graph instantiate failed: %s", hipGetErrorString(e4));
+    }
+    return FSLBM_OK;
+}
+
+int step_impl(fslbm_batch *b, int nsteps)
+{
+    if (b->dirty) {
+        const int rc = rebuild(b);
+        if (rc) return rc;
+    }
+    bool any = false;
+    for (int c = 0; c < NCLASS; c++) any |= b->nwork[c] > 0;
+    int s = 0;
+    if (any && b->use_graph && nsteps >= 2) {
+        if (!b->gexec[0]) {
+            const int rc = capture_graphs(b);
+            if (rc) return rc;
+        }
+        for (; s + 2 <= nsteps; s += 2)
+            BCHK(b, hipGraphLaunch(b->gexec[b->flip], b->stream));
+    }
+    for (; s < nsteps; s++) {
+        if (any) BCHK(b, launch_step_all(b, b->flip));
+        b->flip ^= 1;
+    }
+    for (auto &d : b->doms)
+        if (d.active) d.steps += nsteps;
+    return FSLBM_OK;
+}
+
+size_t domain_bytes(int precision, int nx, int ny, int nz, bool masked)
+{
+    const size_t N = (size_t)nx * ny * nz, nxy = (size_t)nx * ny;
+    const size_t rb = precision == FSLBM_F64 ? 8 : 4;
+    return 2 * 4 * N * storage_bytes(precision) + (masked ? nxy + 3 * nxy * rb : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fslbm_batch_create(int precision, int d1, size_t hbm_budget, fslbm_batch_t *out)
+{
+    g_batch_err.clear();
+    if (!out) return FSLBM_EINVAL;
+    *out = nullptr;
+    if (precision != FSLBM_F32 && precision != FSLBM_F16S && precision != FSLBM_BF16S) {
+        g_batch_err = "batches take FP32 or 16-bit shifted storage";
+        return FSLBM_EINVAL;
+    }
+    if (d1 != 1 && d1 != 2) {
+        g_batch_err = "batches take d1 = 1 or 2";
+        return FSLBM_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_batch_err = "no HIP device visible";
+        return FSLBM_ENODEV;
+    }
+    fslbm_batch *b = new fslbm_batch();
+    b->precision = precision;
+    b->d1 = d1;
+    b->budget = hbm_budget;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        g_batch_err = "stream/event creation failed";
+        fslbm_batch_destroy(b);
+        return FSLBM_EHIP;
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        b->grid = 8 * std::max(1, prop.multiProcessorCount / 8);
+    *out = b;
+    return FSLBM_OK;
+}
+
+void fslbm_batch_destroy(fslbm_batch_t b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (auto &d : b->doms) {
+        for (void *p : d.buf) if (p) (void)hipFree(p);
+        if (d.mask2d) (void)hipFree(d.mask2d);
+        if (d.omega) (void)hipFree(d.omega);
+    }
+    if (b->d_doms) (void)hipFree(b->d_doms);
+    for (auto *w : b->d_work) if (w) (void)hipFree(w);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *fslbm_batch_last_error(fslbm_batch_t b)
+{
+    return b ? b->err.c_str() : g_batch_err.c_str();
+}
+
+long long fslbm_batch_domain_bytes(int precision, const uint8_t *mask, int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return FSLBM_EINVAL;
+    return (long long)domain_bytes(precision, nx, ny, nz, mask != nullptr);
+}
+
+int fslbm_batch_add(fslbm_batch_t b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                    int ny, int nz, const double *p, const double *ux, const double *uy,
+                    const double *uz)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (!prm || !p || !ux || !uy || !uz) return bfail(b, FSLBM_EINVAL, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return bfail(b, FSLBM_EINVAL, "bad box");
+    const long long N = (long long)nx * ny * nz, nxy = (long long)nx * ny;
+    if (N * 4 >= (1ll << 31)) return bfail(b, FSLBM_EINVAL, "domain too large");
+    if (prm->d1 != b->d1) return bfail(b, FSLBM_EINVAL, "domain d1 %d differs from the batch's %d", prm->d1, b->d1);
+    if (!std::isfinite(prm->tblend) || !std::isfinite(prm->pin) || !std::isfinite(prm->nu) ||
+        !std::isfinite(prm->blend))
+        return bfail(b, FSLBM_EINVAL, "non-finite run parameter");
+    Geo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.N = N;
+    g.periodic = mask == nullptr;
+    if (g.periodic && (2 * prm->d1 > nx || 2 * prm->d1 > ny))
+        return bfail(b, FSLBM_EINVAL, "periodic box narrower than the corrector reach");
+    std::vector<uint8_t> m2;
+    if (mask) {
+        for (long long i = 0; i < nxy; i++) {
+            const uint8_t m = mask[i];
+            if (m != 0 && m != 125) return bfail(b, FSLBM_EINVAL, "mask codes must be 0 or 125");
+            const int x = (int)(i % nx), y = (int)(i / nx);
+            if (m == 0 && (x == 0 || y == 0 || x == nx - 1 || y == ny - 1))
+                return bfail(b, FSLBM_EINVAL, "fluid cell on the x/y face; a wall frame is required");
+        }
+        for (int z = 1; z < nz; z++)
+            if (std::memcmp(mask, mask + z * nxy, nxy) != 0)
+                return bfail(b, FSLBM_EINVAL, "batched domains need a mask that is the same in every z plane");
+        m2.assign(mask, mask + nxy);
+    }
+    if (!tiled_supported(g, b->precision, prm->d1, true))
+        return bfail(b, FSLBM_EINVAL, "domain not supported by the tiled kernel (nz >= 16 required)");
+    const size_t bytes = domain_bytes(b->precision, nx, ny, nz, mask != nullptr);
+    if (b->used + bytes > b->budget)
+        return bfail(b, FSLBM_EBUSY, "HBM budget: %zu bytes in use, %zu needed, budget %zu",
+                     b->used, bytes, b->budget);
+
+    Domain d;
+    d.active = true;
+    d.g = g;
+    d.prm = *prm;
+    d.bytes = bytes;
+    d.hmask2d = m2;
+    HostMask hm;
+    hm.m = mask ? d.hmask2d.data() : nullptr;
+    hm.nxy = nxy;
+    hm.plane = true;
+    const double *src[4] = {p, ux, uy, uz};
+    if (b->precision != FSLBM_F32) choose_quant(g, hm, prm->pin, src, d.q);
+    const size_t sbytes = 4 * (size_t)N * storage_bytes(b->precision);
+    std::vector<unsigned char> host(sbytes);
+    encode_state(g, hm, b->precision, d.q, src, host.data());
+    auto cleanup = [&]() {
+        for (void *q : d.buf) if (q) (void)hipFree(q);
+        if (d.mask2d) (void)hipFree(d.mask2d);
+        if (d.omega) (void)hipFree(d.omega);
+    };
+    bool ok = hipMalloc(&d.buf[0], sbytes) == hipSuccess &&
+              hipMalloc(&d.buf[1], sbytes) == hipSuccess;
+    if (ok && mask) {
+        auto om = wall_weights<float>(nx, ny, d.hmask2d.data());
+        ok = hipMalloc((void **)&d.mask2d, nxy) == hipSuccess &&
+             hipMalloc(&d.omega, om.size() * sizeof(float)) == hipSuccess &&
+             hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipMemcpyAsync(d.omega, om.data(), om.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipStreamSynchronize(b->stream) == hipSuccess;
+    }
+    if (ok)
+        ok = hipMemcpyAsync(d.buf[0], host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipMemcpyAsync(d.buf[1], host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipStreamSynchronize(b->stream) == hipSuccess;
+    if (!ok) {
+        cleanup();
+        return bfail(b, FSLBM_ENOMEM, "device allocation for the domain failed");
+    }
+    // reuse a free slot
+    int slot = -1;
+    for (size_t i = 0; i < b->doms.size(); i++)
+        if (!b->doms[i].active) { slot = (int)i; break; }
+    // a new domain starts at the batch's current flip, so that after the
+    // next rebuild its parity (0 ^ flip folded) names buf[0]
+    d.parity = b->flip;
+    if (slot < 0) {
+        slot = (int)b->doms.size();
+        b->doms.push_back(std::move(d));
+    } else {
+        b->doms[slot] = std::move(d);
+    }
+    b->used += bytes;
+    b->dirty = true;
+    return slot;
+}
+
+int fslbm_batch_remove(fslbm_batch_t b, int slot)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
+        return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
+    BCHK(b, hipStreamSynchronize(b->stream));
+    Domain &d = b->doms[slot];
+    for (void *&q : d.buf) if (q) { (void)hipFree(q); q = nullptr; }
+    if (d.mask2d) (void)hipFree(d.mask2d);
+    if (d.omega) (void)hipFree(d.omega);
+    d.mask2d = nullptr;
+    d.omega = nullptr;
+    b->used -= d.bytes;
+    d = Domain();
+    b->dirty = true;
+    return FSLBM_OK;
+}
+
+int fslbm_batch_step(fslbm_batch_t b, int nsteps)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (nsteps < 0) return bfail(b, FSLBM_EINVAL, "nsteps < 0");
+    return step_impl(b, nsteps);
+}
+
+int fslbm_batch_get_state(fslbm_batch_t b, int slot, double *p, double *ux, double *uy,
+                          double *uz)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
+        return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
+    if (!p || !ux || !uy || !uz) return bfail(b, FSLBM_EINVAL, "NULL state array");
+    const Domain &d = b->doms[slot];
+    // current buffer: parity ^ flip (a new domain holds its state in both)
+    const int cur = d.parity ^ b->flip;
+    const size_t sbytes = 4 * (size_t)d.g.N * storage_bytes(b->precision);
+    std::vector<unsigned char> host(sbytes);
+    BCHK(b, hipMemcpyAsync(host.data(), d.buf[cur], sbytes, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    HostMask hm;
+    hm.m = d.g.periodic ? nullptr : d.hmask2d.data();
+    hm.nxy = (long long)d.g.nx * d.g.ny;
+    hm.plane = true;
+    double *dst[4] = {p, ux, uy, uz};
+    decode_state(d.g, hm, b->precision, d.q, host.data(), dst);
+    return FSLBM_OK;
+}
+
+int fslbm_batch_query(fslbm_batch_t b, size_t *used, size_t *budget, int *ndomains,
+                      long long *cells)
+{
+    if (!b) return FSLBM_EINVAL;
+    int n = 0;
+    long long c = 0;
+    for (const auto &d : b->doms)
+        if (d.active) { n++; c += d.g.N; }
+    if (used) *used = b->used;
+    if (budget) *budget = b->budget;
+    if (ndomains) *ndomains = n;
+    if (cells) *cells = c;
+    return FSLBM_OK;
+}
+
+int fslbm_batch_use_graph(fslbm_batch_t b, int enable)
+{
+    if (!b) return FSLBM_EINVAL;
+    b->use_graph = enable != 0;
+    drop_graphs(b);
+    return FSLBM_OK;
+}
+
+int fslbm_batch_set_chunk(fslbm_batch_t b, int chunk)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (chunk < 0) return bfail(b, FSLBM_EINVAL, "chunk must be >= 0");
+    b->chunk = chunk;
+    b->dirty = true;
+    return FSLBM_OK;
+}
+
+int fslbm_batch_autotune(fslbm_batch_t b, int trial_steps, int *best_chunk, double *best_ms)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (trial_steps < 1) return bfail(b, FSLBM_EINVAL, "trial_steps must be >= 1");
+    const int cands[] = {0, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    int best = 0;
+    double bms = 1e300;
+    for (int cand : cands) {
+        int rc = rebuild(b, cand);
+        if (rc) return rc;
+        // trial steps all use flip 0: they read the current states and
+        // write the spare buffers, so no domain advances
+        BCHK(b, launch_step_all(b, 0));
+        BCHK(b, hipEventRecord(b->ev0, b->stream));
+        for (int t = 0; t < trial_steps; t++) BCHK(b, launch_step_all(b, 0));
+        BCHK(b, hipEventRecord(b->ev1, b->stream));
+        BCHK(b, hipEventSynchronize(b->ev1));
+        float ms = 0;
+        BCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms / trial_steps < bms) { bms = ms / trial_steps; best = cand; }
+    }
+    b->chunk = best;
+    const int rc = rebuild(b);
+    if (rc) return rc;
+    if (best_chunk) *best_chunk = best;
+    if (best_ms) *best_ms = bms;
+    return FSLBM_OK;
+}
+
+int fslbm_batch_time_steps(fslbm_batch_t b, int nsteps, double *ms)
+{
+    if (!b || !ms) return FSLBM_EINVAL;
+    if (b->dirty) {
+        const int rc = rebuild(b);
+        if (rc) return rc;
+    }
+    if (b->use_graph && !b->gexec[0]) {
+        const int rc = capture_graphs(b);
+        if (rc) return rc;
+    }
+    BCHK(b, hipEventRecord(b->ev0, b->stream));
+    const int rc = step_impl(b, nsteps);
+    if (rc) return rc;
+    BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BCHK(b, hipEventSynchronize(b->ev1));
+    float f = 0;
+    BCHK(b, hipEventElapsedTime(&f, b->ev0, b->ev1));
+    *ms = f;
+    return FSLBM_OK;
+}
+
+int fslbm_batch_sync(fslbm_batch_t b)
+{
+    if (!b) return FSLBM_EINVAL;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return FSLBM_OK;
+}
+
+}  // extern "C"

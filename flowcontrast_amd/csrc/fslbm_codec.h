@@ -1,0 +1,162 @@
+// Host-side conversion between caller arrays (double / float) and the
+// stored state of a precision mode, shared by single contexts and batch
+// domains.  Walls are stored as 0 and read back as 0.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+
+#include "fslbm_internal.h"
+#include "fslbm_store.h"
+
+namespace fslbm {
+
+// Wall lookup over a full mask (nz planes) or a z-invariant plane.
+struct HostMask {
+    const uint8_t *m = nullptr;  // nullptr: periodic box, no walls
+    long long nxy = 0;
+    bool plane = false;          // m holds one plane, repeated in z
+    bool wall(long long i) const
+    {
+        if (!m) return false;
+        return (plane ? m[i % nxy] : m[i]) != 0;
+    }
+};
+
+inline float pow2_at_least(double v)
+{
+    int e;
+    const double m = std::frexp(v, &e);  // v = m * 2^e, m in [0.5, 1)
+    if (m == 0.5) e -= 1;
+    return std::ldexp(1.0f, e);
+}
+
+// Shifted-storage reference and scales from the state: the pressure
+// reference is the mean plus the mean axial gradient -pin/nz (walled
+// domains); scales map the largest magnitude to [512, 1024) and are
+// powers of two, so scaling is exact.
+template <typename T>
+void choose_quant(const Geo &g, const HostMask &hm, double pin, const T *const src[4],
+                  Quant &q)
+{
+    const long long N = g.N;
+    const int nxy = g.nx * g.ny;
+    const double slope = g.periodic ? 0.0 : -pin / g.nz;
+    double sum = 0;
+    long long n = 0;
+    for (long long i = 0; i < N; i++) {
+        if (hm.wall(i)) continue;
+        sum += (double)src[0][i] - slope * (double)(i / nxy);
+        n++;
+    }
+    const double base = n ? sum / (double)n : 0.0;
+    double pmax = 0, umax = 0;
+    for (long long i = 0; i < N; i++) {
+        if (hm.wall(i)) continue;
+        const double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
+        pmax = std::max(pmax, std::fabs(dp));
+        for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
+    }
+    // headroom for growth: at least 1e-3 for velocity, 1e-4 for pressure
+    umax = std::max(umax, 1e-3);
+    pmax = std::max(pmax, std::max(1e-4, std::fabs(pin)));
+    const float su = pow2_at_least(umax) / 1024.0f;
+    const float sp = pow2_at_least(pmax) / 1024.0f;
+    q.su = su;
+    q.inv_su = 1.0f / su;
+    q.sp = sp;
+    q.inv_sp = 1.0f / sp;
+    q.pbase = (float)base;
+    q.pslope = (float)slope;
+}
+
+template <typename T>
+void encode_state(const Geo &g, const HostMask &hm, int precision, const Quant &q,
+                  const T *const src[4], void *dst)
+{
+    const long long N = g.N;
+    const int nxy = g.nx * g.ny;
+    auto run = [&](auto tag) {
+        using ST = decltype(tag);
+        using S = typename ST::S;
+        using R = typename ST::R;
+        S *d = (S *)dst;
+        for (long long i = 0; i < N; i++) {
+            const bool wall = hm.wall(i);
+            const int z = (int)(i / nxy);
+            d[i] = wall ? S(0) : ST::st_p((R)src[0][i], z, q);
+            for (int f = 1; f < 4; f++)
+                d[f * N + i] = wall ? S(0) : ST::st_u((R)src[f][i], q);
+        }
+    };
+    switch (precision) {
+    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
+    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
+    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
+    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
+    }
+}
+
+template <typename T>
+void decode_state(const Geo &g, const HostMask &hm, int precision, const Quant &q,
+                  const void *src, T *const dst[4])
+{
+    const long long N = g.N;
+    const int nxy = g.nx * g.ny;
+    auto run = [&](auto tag) {
+        using ST = decltype(tag);
+        using S = typename ST::S;
+        const S *s = (const S *)src;
+        for (long long i = 0; i < N; i++) {
+            const bool wall = hm.wall(i);
+            const int z = (int)(i / nxy);
+            dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, q);
+            for (int f = 1; f < 4; f++)
+                dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], q);
+        }
+    };
+    switch (precision) {
+    case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
+    case FSLBM_F32: run(Store<FSLBM_F32>{}); break;
+    case FSLBM_F16S: run(Store<FSLBM_F16S>{}); break;
+    case FSLBM_BF16S: run(Store<FSLBM_BF16S>{}); break;
+    }
+}
+
+// Wall weights of a z-invariant mask, per fluid column (x, y):
+// omega0 = sum of W_e, omegax = sum of W_e cx, omegay = sum of W_e cy over
+// the neighbours x+e that are walls (or outside the box).  The z sums of
+// w(cz) are 1 and of w(cz) cz are 0, so omega_z = 0.  They restore the
+// reference's wall rule (own pressure in place of a wall neighbour's,
+// FSLBM.cu:137-143) in the separable predictor of the tiled kernels.
+template <typename T>
+std::vector<T> wall_weights(int nx, int ny, const uint8_t *m2)
+{
+    const double w[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+    const size_t nxy = (size_t)nx * ny;
+    std::vector<T> om(3 * nxy, T(0));
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            const size_t c = x + (size_t)nx * y;
+            if (m2[c] != 0) continue;
+            double o0 = 0, ox = 0, oy = 0;
+            for (int cy = -1; cy <= 1; cy++)
+                for (int cx = -1; cx <= 1; cx++) {
+                    const int xx = x + cx, yy = y + cy;
+                    const bool wall = xx < 0 || yy < 0 || xx >= nx || yy >= ny ||
+                                      m2[xx + (size_t)nx * yy] != 0;
+                    if (!wall) continue;
+                    const double W = w[cx + 1] * w[cy + 1];
+                    o0 += W;
+                    ox += W * cx;
+                    oy += W * cy;
+                }
+            om[c] = (T)o0;
+            om[nxy + c] = (T)ox;
+            om[2 * nxy + c] = (T)oy;
+        }
+    return om;
+}
+
+}  // namespace fslbm

@@ -69,6 +69,27 @@ hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
 int default_chunk(const Geo &g, int precision);
 
+// Batched step over many domains (32-bit arithmetic modes).  `doms` is a
+// device array of per-domain descriptors (batch_dom_bytes each, filled by
+// batch_fill_dom), `work` the packed (domain, work item) table of one
+// kernel class (periodic / d1 / order-4 arms); flip = steps since the
+// descriptors' parities were written, mod 2.
+struct BatchLaunch {
+    const void *doms;
+    const int2 *work;
+    int nwork, flip, grid;
+    bool periodic, fd2;
+    int d1, precision;
+    hipStream_t stream;
+};
+hipError_t launch_step_batch(const BatchLaunch &b);
+size_t batch_dom_bytes(int precision);
+// returns the domain's number of work items
+int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
+                   void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
+                   int parity);
+void batch_set_parity(void *dst, int precision, int parity);
+
 // Whole-field reductions (8 doubles, see fslbm_reduce) into dst (device).
 hipError_t launch_reduce(const Geo &g, int precision, const void *state,
                          const uint8_t *mask, const Quant &q, double *partials,

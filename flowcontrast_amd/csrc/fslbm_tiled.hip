@@ -548,13 +548,16 @@ __device__ __forceinline__ float lane_shift(float src, float edge)
         __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
 }
 
+// One work item (tile x chunk) of the pipelined step.  Called by the
+// single-domain kernel and by the batched kernel; all threads of the
+// workgroup call it with the same w.
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
-__global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
+__device__ __forceinline__ void pipe_item(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
     const typename Store<MODE>::R *__restrict__ omega,
-    TiledParams<typename Store<MODE>::R> P)
+    const TiledParams<typename Store<MODE>::R> &P, const int w)
 {
     using ST = Store<MODE>;
     using R = typename ST::R;
@@ -576,9 +579,6 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     __shared__ FdCell<R> fd[2][FY][FX];
     __shared__ R edge[2][RYP][2][8];  // [side 0: column x0-1 | 1: x0+64]
 
-    const int bid = blockIdx.x;
-    const int w = (bid & 7) * P.per_xcd + (bid >> 3);
-    if (w >= P.nwork) return;
     const int ntile = P.ntx * P.nty;
     const int chunk = w / ntile;
     const int tile = w - chunk * ntile;
@@ -996,6 +996,117 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     }
 }
 
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+__global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
+    const typename Store<MODE>::S *__restrict__ in,
+    typename Store<MODE>::S *__restrict__ out,
+    const uint8_t *__restrict__ mask2d,
+    const typename Store<MODE>::R *__restrict__ omega,
+    TiledParams<typename Store<MODE>::R> P)
+{
+    // XCD-major work order: the 8 XCDs each take a contiguous run
+    const int bid = blockIdx.x;
+    const int w = (bid & 7) * P.per_xcd + (bid >> 3);
+    if (w >= P.nwork) return;
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, w);
+}
+
+// ---------------------------------------------------------------------
+// Batched step: many independent domains advanced by one persistent
+// launch over a packed table of work items (domain, tile x chunk).
+// Every item runs pipe_item, the same code as the single-domain kernel,
+// so a domain's result does not depend on what else is in the batch.
+// ---------------------------------------------------------------------
+template <typename R>
+struct BatchDom {
+    TiledParams<R> P;
+    void *buf[2];           // the domain's two state buffers
+    const uint8_t *mask2d;  // z-invariant mask plane (nullptr: periodic)
+    const R *omega;         // wall weights (nullptr: periodic)
+    int parity;             // buffer holding the state when flip == 0
+    int pad_;
+};
+
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+__global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
+    const BatchDom<typename Store<MODE>::R> *__restrict__ doms,
+    const int2 *__restrict__ work, int nwork, int flip)
+{
+    using S = typename Store<MODE>::S;
+    // XCD-major: XCD x owns items [x*per, (x+1)*per) and its blocks
+    // stride over them.
+    const int gx = gridDim.x >> 3;
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int per = (nwork + 7) >> 3;
+    for (int i = j; i < per; i += gx) {
+        const int w = xcd * per + i;
+        if (w >= nwork) break;
+        const int2 wd = work[w];
+        const BatchDom<typename Store<MODE>::R> &D = doms[wd.x];
+        const int par = D.parity ^ flip;
+        pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
+                                               D.mask2d, D.omega, D.P, wd.y);
+    }
+}
+
+template <int MODE>
+hipError_t launch_batch_mode(const BatchLaunch &b)
+{
+    using R = typename Store<MODE>::R;
+    constexpr int TYP = pipe_rows(MODE);
+    const auto *doms = (const BatchDom<R> *)b.doms;
+    const dim3 grid(b.grid), block((TYP + 3) * 64);
+#define FSLBM_BATCH(PER, DD, F2)                                                        \
+    k_step_batch<MODE, PER, DD, F2, TYP><<<grid, block, 0, b.stream>>>(doms, b.work, \
+                                                                        b.nwork, b.flip)
+    if (b.periodic) {
+        if (b.d1 == 1) { if (b.fd2) FSLBM_BATCH(true, 1, true); else FSLBM_BATCH(true, 1, false); }
+        else { if (b.fd2) FSLBM_BATCH(true, 2, true); else FSLBM_BATCH(true, 2, false); }
+    } else {
+        if (b.d1 == 1) { if (b.fd2) FSLBM_BATCH(false, 1, true); else FSLBM_BATCH(false, 1, false); }
+        else { if (b.fd2) FSLBM_BATCH(false, 2, true); else FSLBM_BATCH(false, 2, false); }
+    }
+#undef FSLBM_BATCH
+    return hipGetLastError();
+}
+
+// Host-side fill of the per-launch parameters of one domain.
+template <int MODE>
+void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const StepArgs &a,
+                 int chunk_req)
+{
+    using R = typename Store<MODE>::R;
+    const StepK<R> k = [&] {
+        if constexpr (MODE == FSLBM_F64) return make_consts_f64(a);
+        else return make_consts_f32(a);
+    }();
+    const int rows = MODE == FSLBM_F64 ? TY : pipe_rows(MODE);
+    P.nx = g.nx;
+    P.ny = g.ny;
+    P.nz = g.nz;
+    P.nxy = g.nx * g.ny;
+    P.ntx = (g.nx + TX - 1) / TX;
+    P.nty = (g.ny + rows - 1) / rows;
+    P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, MODE);
+    if (P.chunk > g.nz) P.chunk = g.nz;
+    P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
+    P.nwork = P.ntx * P.nty * P.nchunks;
+    P.per_xcd = (P.nwork + 7) / 8;
+    P.pin = k.pin;
+    P.gauge = k.gauge;
+    P.c0 = k.c0;
+    P.c1 = k.c1;
+    P.c2 = k.c2;
+    P.g = k.dtddy * k.m;
+    P.tb = k.c;
+    P.cc = k.cc;
+    P.ku = P.tb * P.g * P.c0 + P.cc;
+    P.ks1 = P.tb * P.g * P.c1;
+    P.ks2 = P.tb * P.g * P.c2;
+    P.kx = P.tb / R(216);
+    P.q = a.q;
+}
+
 template <int MODE, bool PERIODIC, int D1>
 hipError_t launch_t(const Geo &g, const void *in, void *out,
                     const TiledAux &aux,
@@ -1023,36 +1134,11 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
 
 template <int MODE>
 hipError_t launch_mode(const Geo &g, const void *in, void *out,
-                       const TiledAux &aux, const StepArgs &a,
-                       const StepK<typename Store<MODE>::R> &k)
+                       const TiledAux &aux, const StepArgs &a)
 {
     using R = typename Store<MODE>::R;
     TiledParams<R> P;
-    P.nx = g.nx;
-    P.ny = g.ny;
-    P.nz = g.nz;
-    P.nxy = g.nx * g.ny;
-    const int rows = MODE == FSLBM_F64 ? TY : pipe_rows(MODE);  // tile rows of the kernel
-    P.ntx = (g.nx + TX - 1) / TX;
-    P.nty = (g.ny + rows - 1) / rows;
-    P.chunk = aux.chunk > 0 ? aux.chunk : default_chunk(g, MODE);
-    if (P.chunk > g.nz) P.chunk = g.nz;
-    P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
-    P.nwork = P.ntx * P.nty * P.nchunks;
-    P.per_xcd = (P.nwork + 7) / 8;
-    P.pin = k.pin;
-    P.gauge = k.gauge;
-    P.c0 = k.c0;
-    P.c1 = k.c1;
-    P.c2 = k.c2;
-    P.g = k.dtddy * k.m;
-    P.tb = k.c;
-    P.cc = k.cc;
-    P.ku = P.tb * P.g * P.c0 + P.cc;
-    P.ks1 = P.tb * P.g * P.c1;
-    P.ks2 = P.tb * P.g * P.c2;
-    P.kx = P.tb / R(216);
-    P.q = a.q;
+    fill_params<MODE>(P, g, a, aux.chunk);
     const bool per = g.periodic != 0;
     if (a.d1 == 1)
         return per ? launch_t<MODE, true, 1>(g, in, out, aux, P, aux.stream)
@@ -1089,14 +1175,54 @@ hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
                              void *out, const TiledAux &aux, const StepArgs &a)
 {
     switch (precision) {
-    case FSLBM_F64:
-        return launch_mode<FSLBM_F64>(g, in, out, aux, a, make_consts_f64(a));
-    case FSLBM_F32:
-        return launch_mode<FSLBM_F32>(g, in, out, aux, a, make_consts_f32(a));
-    case FSLBM_F16S:
-        return launch_mode<FSLBM_F16S>(g, in, out, aux, a, make_consts_f32(a));
-    case FSLBM_BF16S:
-        return launch_mode<FSLBM_BF16S>(g, in, out, aux, a, make_consts_f32(a));
+    case FSLBM_F64: return launch_mode<FSLBM_F64>(g, in, out, aux, a);
+    case FSLBM_F32: return launch_mode<FSLBM_F32>(g, in, out, aux, a);
+    case FSLBM_F16S: return launch_mode<FSLBM_F16S>(g, in, out, aux, a);
+    case FSLBM_BF16S: return launch_mode<FSLBM_BF16S>(g, in, out, aux, a);
+    }
+    return hipErrorInvalidValue;
+}
+
+size_t batch_dom_bytes(int precision)
+{
+    return precision == FSLBM_F64 ? sizeof(BatchDom<double>) : sizeof(BatchDom<float>);
+}
+
+int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
+                   void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
+                   int parity)
+{
+    auto fill = [&](auto *D, auto tag) -> int {
+        constexpr int M = decltype(tag)::value;
+        fill_params<M>(D->P, g, a, chunk);
+        D->buf[0] = buf0;
+        D->buf[1] = buf1;
+        D->mask2d = mask2d;
+        D->omega = (decltype(D->omega))omega;
+        D->parity = parity;
+        D->pad_ = 0;
+        return D->P.nwork;
+    };
+    switch (precision) {
+    case FSLBM_F32: return fill((BatchDom<float> *)dst, std::integral_constant<int, FSLBM_F32>{});
+    case FSLBM_F16S: return fill((BatchDom<float> *)dst, std::integral_constant<int, FSLBM_F16S>{});
+    case FSLBM_BF16S: return fill((BatchDom<float> *)dst, std::integral_constant<int, FSLBM_BF16S>{});
+    }
+    return -1;
+}
+
+void batch_set_parity(void *dst, int precision, int parity)
+{
+    if (precision == FSLBM_F64) ((BatchDom<double> *)dst)->parity = parity;
+    else ((BatchDom<float> *)dst)->parity = parity;
+}
+
+hipError_t launch_step_batch(const BatchLaunch &b)
+{
+    switch (b.precision) {
+    case FSLBM_F32: return launch_batch_mode<FSLBM_F32>(b);
+    case FSLBM_F16S: return launch_batch_mode<FSLBM_F16S>(b);
+    case FSLBM_BF16S: return launch_batch_mode<FSLBM_BF16S>(b);
     }
     return hipErrorInvalidValue;
 }

@@ -46,6 +46,7 @@ extern "C" {
 #define FSLBM_EHIP (-3)     /* HIP runtime error */
 #define FSLBM_ENODEV (-4)   /* no usable gfx950 device */
 #define FSLBM_ESTATE (-5)   /* call not valid in the context's state */
+#define FSLBM_EBUSY (-6)    /* batch admission refused: HBM budget full */
 
 /* storage / arithmetic modes */
 #define FSLBM_F64 0   /* double storage, double arithmetic, exact rationals */
@@ -230,6 +231,56 @@ int fslbm_init_taylor_green(int nx, int ny, int nz, double u0, double rho0,
  * first/last fluid planes. */
 int fslbm_fold_mask(int lx, int ly, int lz, int d2, const uint8_t *padded,
                     uint8_t *folded);
+
+/* ------------------------------------------------------------------ */
+/* Batched domains (no counterpart in the reference, which runs one     */
+/* hard-coded pipe): many independent pipes / periodic boxes advanced   */
+/* together, one persistent launch per step over a packed table of      */
+/* (domain, tile x z-chunk) work items.  Every work item runs the same  */
+/* device code as the single-domain tiled kernel, so a domain's result  */
+/* is bitwise the same alone or in any mix.  Domains are admitted       */
+/* against an HBM budget and stay device-resident until removed.        */
+/* Precision FSLBM_F32, FSLBM_F16S or FSLBM_BF16S; one d1 (1 or 2) per  */
+/* batch; masks must be the same in every z plane.                      */
+/* ------------------------------------------------------------------ */
+typedef struct fslbm_batch *fslbm_batch_t;
+
+int fslbm_batch_create(int precision, int d1, size_t hbm_budget,
+                       fslbm_batch_t *out);
+void fslbm_batch_destroy(fslbm_batch_t b);
+const char *fslbm_batch_last_error(fslbm_batch_t b);
+
+/* Device bytes a domain of this size takes in a batch (admission test). */
+long long fslbm_batch_domain_bytes(int precision, const uint8_t *mask, int nx,
+                                   int ny, int nz);
+
+/* Admit a domain with its run parameters, mask (NULL: periodic box) and
+ * initial state (four arrays of nx*ny*nz doubles).  Returns the slot id
+ * (>= 0), FSLBM_EBUSY when the budget cannot hold it, or another error. */
+int fslbm_batch_add(fslbm_batch_t b, const fslbm_params *prm,
+                    const uint8_t *mask, int nx, int ny, int nz,
+                    const double *p, const double *ux, const double *uy,
+                    const double *uz);
+int fslbm_batch_remove(fslbm_batch_t b, int slot);
+
+/* Advance every admitted domain by nsteps (asynchronous). */
+int fslbm_batch_step(fslbm_batch_t b, int nsteps);
+int fslbm_batch_get_state(fslbm_batch_t b, int slot, double *p, double *ux,
+                          double *uy, double *uz);
+int fslbm_batch_query(fslbm_batch_t b, size_t *used, size_t *budget,
+                      int *ndomains, long long *cells);
+
+/* Replay the step sequence from a captured hipGraph (two steps per
+ * graph launch). */
+int fslbm_batch_use_graph(fslbm_batch_t b, int enable);
+
+/* Schedule: z planes per work item (0: per-domain default), or pick the
+ * fastest by timing trial launches that do not advance any domain. */
+int fslbm_batch_set_chunk(fslbm_batch_t b, int chunk);
+int fslbm_batch_autotune(fslbm_batch_t b, int trial_steps, int *best_chunk,
+                         double *best_ms);
+int fslbm_batch_time_steps(fslbm_batch_t b, int nsteps, double *ms);
+int fslbm_batch_sync(fslbm_batch_t b);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,111 @@
+"""GPU tests of batched domains: a domain gives bitwise the same state
+alone (single context, tiled kernel) and inside a batch, whatever else
+the batch holds, with or without hipGraph replay, and when domains are
+admitted or removed while others run; admission respects the HBM
+budget."""
+import numpy as np
+import pytest
+
+from flowcontrast_amd import (KERNEL_TILED, Batch, BudgetFull, RunParams, Solver,
+                              inputs)
+
+pytestmark = pytest.mark.gpu
+
+
+def pipe_case(n, nz, R, re, nu, seed, blend=0.0, tb=1.0):
+    mask, _ = inputs.circle_mask(n, n, nz, R)
+    ubar = re * nu / (2 * R)
+    pin = inputs.laminar_pin(nu, nz, ubar, R)
+    st = inputs.poiseuille_state(mask, R, ubar, pin, 0.02, seed)
+    return RunParams(tblend=tb, pin=pin, nu=nu, d1=2, delta=1, blend=blend), mask, st
+
+
+def tgv_case(n):
+    nu = inputs.tgv_viscosity(n, 0.05, 800.0)
+    return RunParams(tblend=1.0, pin=0.0, nu=nu, d1=2, delta=1, blend=0.0), None, \
+        inputs.taylor_green_state(n, 0.05, 1.0)
+
+
+def alone(prm, mask, st, steps, precision):
+    shape = np.shape(st[0])
+    with Solver(shape, prm, mask=mask, precision=precision, kernel=KERNEL_TILED) as s:
+        s.set_state(*st)
+        s.step(steps)
+        return s.get_state()
+
+
+CASES = [pipe_case(30, 64, 12.5, 300, 0.01, 1),
+         pipe_case(52, 40, 23.0, 900, 0.008, 2, blend=0.5, tb=0.9),
+         pipe_case(100, 24, 47.0, 2500, 0.005, 3),
+         tgv_case(32)]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16s"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_equals_alone(precision, graph):
+    steps = 13
+    ref = [alone(prm, mask, st, steps, precision) for prm, mask, st in CASES]
+    with Batch(precision, 2, 8 << 30) as b:
+        b.use_graph(graph)
+        slots = [b.add(prm, st, mask) for prm, mask, st in CASES]
+        b.step(steps)
+        for slot, r in zip(slots, ref):
+            got = b.get_state(slot)
+            for g, e in zip(got, r):
+                assert np.array_equal(g, e)
+
+
+def test_admission_and_removal_mid_run():
+    """Domains admitted later or next to removed ones still match a solo
+    run of the same number of steps."""
+    prmA, mA, sA = CASES[0]
+    prmB, mB, sB = CASES[2]
+    prmC, mC, sC = CASES[3]
+    refA = alone(prmA, mA, sA, 20, "f32")
+    refB = alone(prmB, mB, sB, 7, "f32")
+    refC = alone(prmC, mC, sC, 11, "f32")
+    with Batch("f32", 2, 8 << 30) as b:
+        b.use_graph(True)
+        a = b.add(prmA, sA, mA)
+        b.step(9)
+        c = b.add(prmC, sC, mC)
+        b.step(4)
+        bb = b.add(prmB, sB, mB)
+        b.step(7)
+        assert all(np.array_equal(g, e) for g, e in zip(b.get_state(c), refC))
+        b.remove(c)
+        assert all(np.array_equal(g, e) for g, e in zip(b.get_state(bb), refB))
+        b.remove(bb)
+        assert all(np.array_equal(g, e) for g, e in zip(b.get_state(a), refA))
+        assert b.query()["domains"] == 1
+
+
+def test_hbm_budget_admission():
+    prm, mask, st = CASES[2]
+    need = Batch.domain_bytes("f32", np.shape(st[0]), True)
+    with Batch("f32", 2, int(need * 2.5)) as b:
+        s0 = b.add(prm, st, mask)
+        s1 = b.add(prm, st, mask)
+        with pytest.raises(BudgetFull):
+            b.add(prm, st, mask)
+        assert b.query()["used"] == 2 * need
+        b.remove(s0)
+        s2 = b.add(prm, st, mask)  # memory freed: admitted into the free slot
+        assert s2 == s0 and b.query()["domains"] == 2
+        b.remove(s1)
+        b.remove(s2)
+        assert b.query()["used"] == 0
+
+
+def test_batch_autotune_keeps_state():
+    with Batch("f32", 2, 8 << 30) as b:
+        slots = [b.add(prm, st, mask) for prm, mask, st in CASES]
+        before = [b.get_state(s) for s in slots]
+        chunk, ms = b.autotune(2)
+        assert ms > 0
+        for s, bef in zip(slots, before):
+            assert all(np.array_equal(g, e) for g, e in zip(b.get_state(s), bef))
+        b.step(5)
+        for (prm, mask, st), s in zip(CASES, slots):
+            ref = alone(prm, mask, st, 5, "f32")
+            assert all(np.array_equal(g, e) for g, e in zip(b.get_state(s), ref))
