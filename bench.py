@@ -90,6 +90,175 @@ def copy_calibration() -> int:
     return 0
 
 
+def rel_l2(a, b) -> float:
+    den = float(np.sqrt(sum(np.sum(np.square(x)) for x in b)))
+    num = float(np.sqrt(sum(np.sum(np.square(x - y)) for x, y in zip(a, b))))
+    return num / den if den > 0 else num
+
+
+def field_errors(got, ref):
+    """Pointwise: relative L2 of the velocity vector field and of the
+    pressure deviation from its mean (p is a large constant plus O(u^2)).
+    Integral: kinetic energy sum |u|^2 and mean axial velocity.  Past the
+    predictability time of a transitional flow the pointwise figures
+    measure chaos, the integral ones the solver."""
+    gp, rp = got[0] - got[0].mean(), ref[0] - ref[0].mean()
+    ke_g = sum(float(np.sum(np.square(a))) for a in got[1:])
+    ke_r = sum(float(np.sum(np.square(a))) for a in ref[1:])
+    uz_g, uz_r = float(np.mean(got[3])), float(np.mean(ref[3]))
+    return {"u_rel_l2": rel_l2(got[1:], ref[1:]), "p_rel_l2": rel_l2([gp], [rp]),
+            "ke_rel": abs(ke_g - ke_r) / ke_r if ke_r else abs(ke_g),
+            "mean_uz_rel": abs(uz_g - uz_r) / abs(uz_r) if uz_r else abs(uz_g)}
+
+
+def bench_batch(count: int, steps: int, sequential: bool):
+    """Independent uneven pipes in one batch (persistent launch per step,
+    autotuned chunk, hipGraph replay) against the same pipes solved one
+    after another by the single-domain solver.  Timed: the steps only,
+    states resident on the device on both sides."""
+    from flowcontrast_amd import KERNEL_AUTO, Batch, Solver
+    from flowcontrast_amd import workloads as W
+    cases = W.uneven_pipes(count, 4, steps)
+    cells = sum(c.cells for c in cases)
+    print(f"bench: batch of {count} pipes, {cells} cells, {steps} steps", file=sys.stderr,
+          flush=True)
+    t = time.perf_counter()
+    with Batch("f32", 2, 256 << 30) as b:
+        slots = []
+        for c in cases:
+            prm, mask, st = c.build()
+            slots.append(b.add(prm, st, mask))
+        upload = time.perf_counter() - t
+        b.use_graph(True)
+        chunk, _ = b.autotune(2)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        wall = time.perf_counter() - t0
+        finite = all(np.isfinite(a).all() for sl in slots[:4] for a in b.get_state(sl))
+    out = {"workload": f"{count} periodic pipes, radius U[12,48], length U[4,10] D, "
+                       f"Re U[200,3000], Poiseuille + 2% noise, seed 4, FP32",
+           "pipes": count, "cells": cells, "steps_per_solve": steps,
+           "solves_per_s": round(count / wall, 4),
+           "mlups": round(cells * steps / wall / 1e6, 1),
+           "seconds": round(wall, 3), "chunk": chunk, "graph": True,
+           "upload_s": round(upload, 2), "finite": bool(finite)}
+    if sequential:
+        print("bench: the same pipes one after another", file=sys.stderr, flush=True)
+        seq = 0.0
+        for c in cases:
+            prm, mask, st = c.build()
+            with Solver(c.shape, prm, mask=mask, precision="f32", kernel=KERNEL_AUTO) as s:
+                s.set_state(*st)
+                s.sync()
+                t0 = time.perf_counter()
+                s.step(steps)
+                s.sync()
+                seq += time.perf_counter() - t0
+        out.update({"sequential_solves_per_s": round(count / seq, 4),
+                    "sequential_mlups": round(cells * steps / seq / 1e6, 1),
+                    "sequential_seconds": round(seq, 3),
+                    "speedup_vs_sequential": round(seq / wall, 3)})
+    return out
+
+
+def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
+                sequential: bool):
+    """Poisson stream of pipe / Taylor-Green solve requests through the
+    serving loop (continuous batching under an HBM budget).  Latency is
+    time to solution from each request's scheduled arrival to its
+    hand-back.  Then the same requests submitted at once (capacity) and
+    one after another through the single-domain solver; the smallest
+    requests are checked against solo runs (bitwise) and FP64 runs."""
+    from flowcontrast_amd import KERNEL_AUTO, Server, Solver
+    from flowcontrast_amd import workloads as W
+    cases = W.serving_requests(count, 5, rate)
+    built = [c.build() for c in cases]
+    work = sum(c.cells * c.steps for c in cases)
+    spot_ids = set(sorted(range(count), key=lambda k: cases[k].cells * cases[k].steps)[:spot])
+    print(f"bench: serving {count} requests at {rate}/s", file=sys.stderr, flush=True)
+
+    def run(open_loop: bool):
+        lat, status, kept = {}, {}, {}
+        peak_used = peak_queue = 0
+        with Server(d1=2, hbm_budget=budget, quantum=quantum) as srv:
+            t_start = srv.clock()
+            owner, i = {}, 0
+            while len(lat) < count:
+                now = srv.clock() - t_start
+                while i < count and (not open_loop or cases[i].arrival <= now):
+                    prm, mask, st = built[i]
+                    owner[srv.submit(prm, st, mask, cases[i].precision, cases[i].steps)] = i
+                    i += 1
+                q = srv.query()
+                peak_queue = max(peak_queue, q["queued"])
+                if q["queued"] == 0 and q["running"] == 0 and i < count:
+                    time.sleep(max(0.0, cases[i].arrival - now))
+                    continue
+                srv.pump()
+                peak_used = max(peak_used, srv.query()["used"])
+                for c in srv.completions():
+                    k = owner[c.id]
+                    t_arr = cases[k].arrival if open_loop else 0.0
+                    lat[k] = c.t_done - t_start - t_arr
+                    status[k] = c.status
+                    if k in spot_ids and c.ok:
+                        kept[k] = srv.fetch(c.id)
+                    srv.drop(c.id)
+            end = srv.clock() - t_start
+        return lat, status, kept, end, peak_used, peak_queue
+
+    lat, status, kept, end, peak_used, peak_queue = run(True)
+    l = np.array([lat[k] for k in range(count)])
+    first = cases[0].arrival
+    out = {"workload": f"{count} requests, Poisson {rate}/s, 80% pipes (R U[16,64], "
+                       f"L U[4,8] D, Re U[200,6000]) / 20% Taylor-Green (64^3-192^3, "
+                       f"Re U[100,1600]), 2000-20000 steps, FP32 or FP16 storage, seed 5",
+           "requests": count, "rate_per_s": rate, "quantum": quantum,
+           "hbm_budget": budget, "cell_updates": work,
+           "p50_s": round(float(np.percentile(l, 50)), 4),
+           "p99_s": round(float(np.percentile(l, 99)), 4),
+           "max_s": round(float(l.max()), 4),
+           "solves_per_s": round(count / (end - first), 4),
+           "failed": int(sum(1 for v in status.values() if v != 0)),
+           "peak_hbm_bytes": int(peak_used), "peak_queue": int(peak_queue)}
+    _, status2, _, end2, _, _ = run(False)
+    out["capacity_solves_per_s"] = round(count / end2, 4)
+    out["capacity_mlups"] = round(work / end2 / 1e6, 1)
+    checks = []
+    for k in sorted(kept):
+        c = cases[k]
+        prm, mask, st = built[k]
+        res = {"request": k, "kind": c.kind, "shape": list(c.shape), "steps": c.steps,
+               "precision": c.precision}
+        with Solver(c.shape, prm, mask=mask, precision=c.precision, kernel=KERNEL_AUTO) as s:
+            s.set_state(*st)
+            s.step(c.steps)
+            solo = s.get_state()
+        res["equal_to_solo"] = bool(all(np.array_equal(a, b) for a, b in zip(kept[k], solo)))
+        with Solver(c.shape, prm, mask=mask, precision="f64", kernel=KERNEL_AUTO) as s:
+            s.set_state(*st)
+            s.step(c.steps)
+            ref = s.get_state()
+        res.update({k2: float(f"{v:.3e}") for k2, v in field_errors(kept[k], ref).items()})
+        checks.append(res)
+    out["spot_checks_vs_fp64"] = checks
+    if sequential:
+        seq = 0.0
+        for c, (prm, mask, st) in zip(cases, built):
+            with Solver(c.shape, prm, mask=mask, precision=c.precision, kernel=KERNEL_AUTO) as s:
+                s.set_state(*st)
+                s.sync()
+                t0 = time.perf_counter()
+                s.step(c.steps)
+                s.sync()
+                seq += time.perf_counter() - t0
+        out["sequential_solves_per_s"] = round(count / seq, 4)
+        out["capacity_vs_sequential"] = round(seq / end2, 3)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +274,17 @@ def main() -> int:
                          "default, -1 = autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="all", choices=["all", "pipe", "batch", "serve"],
+                    help="all: the pipe step line with the batched and serving "
+                         "results attached")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=10000)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-quantum", type=int, default=200)
+    ap.add_argument("--serve-budget", type=int, default=32 << 30)
+    ap.add_argument("--spot-checks", type=int, default=3)
+    ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB x 10 (the "
                          "scale for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -120,6 +300,16 @@ def main() -> int:
               f"z-slab halo exchange that is not part of this build.",
               file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args.batch_count, args.batch_steps,
+                                     not args.no_sequential)), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args.serve_requests, args.serve_rate, args.serve_quantum,
+                                     args.serve_budget, args.spot_checks,
+                                     not args.no_sequential)), flush=True)
+        return 0
 
     from flowcontrast_amd import KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver
 
@@ -196,6 +386,17 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mask, st, pin, nu, args.cpu_steps,
                                            args.precision)
+    if args.workload == "all":
+        for key, fn in (("batched", lambda: bench_batch(args.batch_count, args.batch_steps,
+                                                       not args.no_sequential)),
+                        ("serving", lambda: bench_serve(args.serve_requests, args.serve_rate,
+                                                        args.serve_quantum, args.serve_budget,
+                                                        args.spot_checks,
+                                                        not args.no_sequential))):
+            try:
+                out[key] = fn()
+            except Exception as e:  # reported in the line, never hidden
+                out[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
     return 0
 

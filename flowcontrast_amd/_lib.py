@@ -35,6 +35,10 @@ EXPORTS = [
     "fslbm_batch_step", "fslbm_batch_get_state", "fslbm_batch_query",
     "fslbm_batch_use_graph", "fslbm_batch_set_chunk", "fslbm_batch_autotune",
     "fslbm_batch_time_steps", "fslbm_batch_sync",
+    "fslbm_server_create", "fslbm_server_destroy", "fslbm_server_last_error",
+    "fslbm_server_clock", "fslbm_server_submit", "fslbm_server_pump",
+    "fslbm_server_next_done", "fslbm_server_fetch", "fslbm_server_drop",
+    "fslbm_server_query",
 ]
 EBUSY = -6
 
@@ -49,6 +53,32 @@ class Params(C.Structure):
         ("blend", C.c_double),
         ("d1", C.c_int),
         ("delta", C.c_int),
+    ]
+
+
+class Request(C.Structure):
+    """fslbm_request: one solve of the serving loop."""
+
+    _fields_ = [
+        ("params", Params),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("nz", C.c_int),
+        ("precision", C.c_int),
+        ("nsteps", C.c_longlong),
+    ]
+
+
+class Completion(C.Structure):
+    """fslbm_completion: a finished (or failed) request."""
+
+    _fields_ = [
+        ("id", C.c_longlong),
+        ("status", C.c_int),
+        ("steps", C.c_longlong),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
     ]
 
 
@@ -124,6 +154,18 @@ def lib() -> C.CDLL:
         "fslbm_batch_autotune": (i, [vp, i, ip, C.POINTER(d)]),
         "fslbm_batch_time_steps": (i, [vp, i, C.POINTER(d)]),
         "fslbm_batch_sync": (i, [vp]),
+        "fslbm_server_create": (i, [i, C.c_size_t, i, C.POINTER(vp)]),
+        "fslbm_server_destroy": (None, [vp]),
+        "fslbm_server_last_error": (C.c_char_p, [vp]),
+        "fslbm_server_clock": (d, [vp]),
+        "fslbm_server_submit": (C.c_longlong, [vp, C.POINTER(Request), C.c_void_p,
+                                               _dp, _dp, _dp, _dp]),
+        "fslbm_server_pump": (i, [vp]),
+        "fslbm_server_next_done": (i, [vp, C.POINTER(Completion)]),
+        "fslbm_server_fetch": (i, [vp, C.c_longlong, _dp, _dp, _dp, _dp]),
+        "fslbm_server_drop": (i, [vp, C.c_longlong]),
+        "fslbm_server_query": (i, [vp, ip, ip, ip, C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -152,5 +194,13 @@ def check_batch(rc: int, b=None) -> int:
     if rc < 0:
         L = lib()
         msg = L.fslbm_batch_last_error(b).decode()
+        raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}", rc)
+    return rc
+
+
+def check_server(rc: int, srv=None) -> int:
+    if rc < 0:
+        L = lib()
+        msg = L.fslbm_server_last_error(srv).decode()
         raise FslbmError(f"{L.fslbm_error_string(rc).decode()}: {msg}", rc)
     return rc

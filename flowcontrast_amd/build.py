@@ -37,6 +37,7 @@ LIB_SOURCES = [
     "fslbm_kernels.hip",
     "fslbm_tiled.hip",
     "fslbm_batch.cpp",
+    "fslbm_serve.cpp",
     "fslbm_api.cpp",
     "fslbm_inputs.cpp",
 ]

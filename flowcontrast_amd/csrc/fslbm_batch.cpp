@@ -233,6 +233,132 @@ size_t domain_bytes(int precision, int nx, int ny, int nz, bool masked)
 
 }  // namespace
 
+namespace fslbm {
+
+// Validate and encode a domain on the host (no device work; safe while
+// the batch's launches run).  The mask (NULL: periodic box) is one z plane.
+int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                  int ny, int nz, const double *const src[4], PreparedDomain &pd)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (!prm || !src[0] || !src[1] || !src[2] || !src[3])
+        return bfail(b, FSLBM_EINVAL, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return bfail(b, FSLBM_EINVAL, "bad box");
+    const long long N = (long long)nx * ny * nz, nxy = (long long)nx * ny;
+    if (N * 4 >= (1ll << 31)) return bfail(b, FSLBM_EINVAL, "domain too large");
+    if (prm->d1 != b->d1) return bfail(b, FSLBM_EINVAL, "domain d1 %d differs from the batch's %d", prm->d1, b->d1);
+    if (!std::isfinite(prm->tblend) || !std::isfinite(prm->pin) || !std::isfinite(prm->nu) ||
+        !std::isfinite(prm->blend))
+        return bfail(b, FSLBM_EINVAL, "non-finite run parameter");
+    Geo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.N = N;
+    g.periodic = mask == nullptr;
+    if (g.periodic && (2 * prm->d1 > nx || 2 * prm->d1 > ny))
+        return bfail(b, FSLBM_EINVAL, "periodic box narrower than the corrector reach");
+    if (mask) {
+        for (long long i = 0; i < nxy; i++) {
+            const uint8_t m = mask[i];
+            if (m != 0 && m != 125) return bfail(b, FSLBM_EINVAL, "mask codes must be 0 or 125");
+            const int x = (int)(i % nx), y = (int)(i / nx);
+            if (m == 0 && (x == 0 || y == 0 || x == nx - 1 || y == ny - 1))
+                return bfail(b, FSLBM_EINVAL, "fluid cell on the x/y face; a wall frame is required");
+        }
+    }
+    if (!tiled_supported(g, b->precision, prm->d1, true))
+        return bfail(b, FSLBM_EINVAL, "domain not supported by the tiled kernel (nz >= 16 required)");
+    pd = PreparedDomain();
+    pd.g = g;
+    pd.prm = *prm;
+    pd.precision = b->precision;
+    pd.bytes = domain_bytes(b->precision, nx, ny, nz, mask != nullptr);
+    if (mask) pd.mask2d.assign(mask, mask + nxy);
+    HostMask hm;
+    hm.m = mask ? pd.mask2d.data() : nullptr;
+    hm.nxy = nxy;
+    hm.plane = true;
+    if (b->precision != FSLBM_F32) choose_quant(g, hm, prm->pin, src, pd.q);
+    pd.host.resize(4 * (size_t)N * storage_bytes(b->precision));
+    encode_state(g, hm, b->precision, pd.q, src, pd.host.data());
+    if (mask) pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
+    return FSLBM_OK;
+}
+
+// Upload a prepared domain and admit it.  Returns the slot, FSLBM_EBUSY
+// when the budget cannot hold it, or another error.
+int batch_attach(fslbm_batch *b, PreparedDomain &pd)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (pd.precision != b->precision)
+        return bfail(b, FSLBM_EINVAL, "domain was prepared for another precision mode");
+    const size_t bytes = pd.bytes;
+    if (b->used + bytes > b->budget)
+        return bfail(b, FSLBM_EBUSY, "HBM budget: %zu bytes in use, %zu needed, budget %zu",
+                     b->used, bytes, b->budget);
+    const Geo &g = pd.g;
+    const long long nxy = (long long)g.nx * g.ny;
+    Domain d;
+    d.active = true;
+    d.g = g;
+    d.prm = pd.prm;
+    d.q = pd.q;
+    d.bytes = bytes;
+    d.hmask2d = std::move(pd.mask2d);
+    const size_t sbytes = pd.host.size();
+    auto cleanup = [&]() {
+        for (void *q : d.buf) if (q) (void)hipFree(q);
+        if (d.mask2d) (void)hipFree(d.mask2d);
+        if (d.omega) (void)hipFree(d.omega);
+    };
+    bool ok = hipMalloc(&d.buf[0], sbytes) == hipSuccess &&
+              hipMalloc(&d.buf[1], sbytes) == hipSuccess;
+    if (ok && !g.periodic) {
+        ok = hipMalloc((void **)&d.mask2d, nxy) == hipSuccess &&
+             hipMalloc(&d.omega, pd.omega.size() * sizeof(float)) == hipSuccess &&
+             hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipMemcpyAsync(d.omega, pd.omega.data(), pd.omega.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) == hipSuccess;
+    }
+    if (ok)
+        ok = hipMemcpyAsync(d.buf[0], pd.host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipMemcpyAsync(d.buf[1], pd.host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
+             hipStreamSynchronize(b->stream) == hipSuccess;
+    if (!ok) {
+        cleanup();
+        return bfail(b, FSLBM_ENOMEM, "device allocation for the domain failed");
+    }
+    pd.host = std::vector<unsigned char>();
+    pd.omega = std::vector<float>();
+    // reuse a free slot
+    int slot = -1;
+    for (size_t i = 0; i < b->doms.size(); i++)
+        if (!b->doms[i].active) { slot = (int)i; break; }
+    // a new domain starts at the batch's current flip, so that after the
+    // next rebuild its parity (0 ^ flip folded) names buf[0]
+    d.parity = b->flip;
+    if (slot < 0) {
+        slot = (int)b->doms.size();
+        b->doms.push_back(std::move(d));
+    } else {
+        b->doms[slot] = std::move(d);
+    }
+    b->used += bytes;
+    b->dirty = true;
+    return slot;
+}
+
+int batch_add_plane(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                    int ny, int nz, const double *const src[4])
+{
+    PreparedDomain pd;
+    const int rc = batch_prepare(b, prm, mask, nx, ny, nz, src, pd);
+    if (rc) return rc;
+    return batch_attach(b, pd);
+}
+
+}  // namespace fslbm
+
 extern "C" {
 
 int fslbm_batch_create(int precision, int d1, size_t hbm_budget, fslbm_batch_t *out)
@@ -307,95 +433,13 @@ int fslbm_batch_add(fslbm_batch_t b, const fslbm_params *prm, const uint8_t *mas
     if (!b) return FSLBM_EINVAL;
     if (!prm || !p || !ux || !uy || !uz) return bfail(b, FSLBM_EINVAL, "NULL argument");
     if (nx < 1 || ny < 1 || nz < 1) return bfail(b, FSLBM_EINVAL, "bad box");
-    const long long N = (long long)nx * ny * nz, nxy = (long long)nx * ny;
-    if (N * 4 >= (1ll << 31)) return bfail(b, FSLBM_EINVAL, "domain too large");
-    if (prm->d1 != b->d1) return bfail(b, FSLBM_EINVAL, "domain d1 %d differs from the batch's %d", prm->d1, b->d1);
-    if (!std::isfinite(prm->tblend) || !std::isfinite(prm->pin) || !std::isfinite(prm->nu) ||
-        !std::isfinite(prm->blend))
-        return bfail(b, FSLBM_EINVAL, "non-finite run parameter");
-    Geo g;
-    g.nx = nx;
-    g.ny = ny;
-    g.nz = nz;
-    g.N = N;
-    g.periodic = mask == nullptr;
-    if (g.periodic && (2 * prm->d1 > nx || 2 * prm->d1 > ny))
-        return bfail(b, FSLBM_EINVAL, "periodic box narrower than the corrector reach");
-    std::vector<uint8_t> m2;
-    if (mask) {
-        for (long long i = 0; i < nxy; i++) {
-            const uint8_t m = mask[i];
-            if (m != 0 && m != 125) return bfail(b, FSLBM_EINVAL, "mask codes must be 0 or 125");
-            const int x = (int)(i % nx), y = (int)(i / nx);
-            if (m == 0 && (x == 0 || y == 0 || x == nx - 1 || y == ny - 1))
-                return bfail(b, FSLBM_EINVAL, "fluid cell on the x/y face; a wall frame is required");
-        }
+    const long long nxy = (long long)nx * ny;
+    if (mask)
         for (int z = 1; z < nz; z++)
             if (std::memcmp(mask, mask + z * nxy, nxy) != 0)
                 return bfail(b, FSLBM_EINVAL, "batched domains need a mask that is the same in every z plane");
-        m2.assign(mask, mask + nxy);
-    }
-    if (!tiled_supported(g, b->precision, prm->d1, true))
-        return bfail(b, FSLBM_EINVAL, "domain not supported by the tiled kernel (nz >= 16 required)");
-    const size_t bytes = domain_bytes(b->precision, nx, ny, nz, mask != nullptr);
-    if (b->used + bytes > b->budget)
-        return bfail(b, FSLBM_EBUSY, "HBM budget: %zu bytes in use, %zu needed, budget %zu",
-                     b->used, bytes, b->budget);
-
-    Domain d;
-    d.active = true;
-    d.g = g;
-    d.prm = *prm;
-    d.bytes = bytes;
-    d.hmask2d = m2;
-    HostMask hm;
-    hm.m = mask ? d.hmask2d.data() : nullptr;
-    hm.nxy = nxy;
-    hm.plane = true;
     const double *src[4] = {p, ux, uy, uz};
-    if (b->precision != FSLBM_F32) choose_quant(g, hm, prm->pin, src, d.q);
-    const size_t sbytes = 4 * (size_t)N * storage_bytes(b->precision);
-    std::vector<unsigned char> host(sbytes);
-    encode_state(g, hm, b->precision, d.q, src, host.data());
-    auto cleanup = [&]() {
-        for (void *q : d.buf) if (q) (void)hipFree(q);
-        if (d.mask2d) (void)hipFree(d.mask2d);
-        if (d.omega) (void)hipFree(d.omega);
-    };
-    bool ok = hipMalloc(&d.buf[0], sbytes) == hipSuccess &&
-              hipMalloc(&d.buf[1], sbytes) == hipSuccess;
-    if (ok && mask) {
-        auto om = wall_weights<float>(nx, ny, d.hmask2d.data());
-        ok = hipMalloc((void **)&d.mask2d, nxy) == hipSuccess &&
-             hipMalloc(&d.omega, om.size() * sizeof(float)) == hipSuccess &&
-             hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipMemcpyAsync(d.omega, om.data(), om.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipStreamSynchronize(b->stream) == hipSuccess;
-    }
-    if (ok)
-        ok = hipMemcpyAsync(d.buf[0], host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipMemcpyAsync(d.buf[1], host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipStreamSynchronize(b->stream) == hipSuccess;
-    if (!ok) {
-        cleanup();
-        return bfail(b, FSLBM_ENOMEM, "device allocation for the domain failed");
-    }
-    // reuse a free slot
-    int slot = -1;
-    for (size_t i = 0; i < b->doms.size(); i++)
-        if (!b->doms[i].active) { slot = (int)i; break; }
-    // a new domain starts at the batch's current flip, so that after the
-    // next rebuild its parity (0 ^ flip folded) names buf[0]
-    d.parity = b->flip;
-    if (slot < 0) {
-        slot = (int)b->doms.size();
-        b->doms.push_back(std::move(d));
-    } else {
-        b->doms[slot] = std::move(d);
-    }
-    b->used += bytes;
-    b->dirty = true;
-    return slot;
+    return batch_add_plane(b, prm, mask, nx, ny, nz, src);
 }
 
 int fslbm_batch_remove(fslbm_batch_t b, int slot)
@@ -537,3 +581,28 @@ int fslbm_batch_sync(fslbm_batch_t b)
 }
 
 }  // extern "C"
+
+namespace fslbm {
+
+// Copy a domain's current stored state to the host and remove it from the
+// batch (the serving loop's hand-back of a finished solve).
+int batch_take(fslbm_batch *b, int slot, HostResult &out)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
+        return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
+    const Domain &d = b->doms[slot];
+    const int cur = d.parity ^ b->flip;
+    const size_t sbytes = 4 * (size_t)d.g.N * storage_bytes(b->precision);
+    out.g = d.g;
+    out.precision = b->precision;
+    out.q = d.q;
+    out.mask2d = d.hmask2d;
+    out.steps = d.steps;
+    out.raw.resize(sbytes);
+    BCHK(b, hipMemcpyAsync(out.raw.data(), d.buf[cur], sbytes, hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return fslbm_batch_remove(b, slot);
+}
+
+}  // namespace fslbm

@@ -6,6 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
+#include <thread>
+#include <vector>
 
 #include "fslbm_internal.h"
 #include "fslbm_store.h"
@@ -23,6 +26,42 @@ struct HostMask {
         return (plane ? m[i % nxy] : m[i]) != 0;
     }
 };
+
+// Host threads for the codec loops: OMP_NUM_THREADS if set, else the
+// hardware's count, at most 16.
+inline int codec_threads()
+{
+    static const int n = [] {
+        int t = 0;
+        if (const char *e = std::getenv("OMP_NUM_THREADS")) t = std::atoi(e);
+        if (t <= 0) t = (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(t, 16));
+    }();
+    return n;
+}
+
+// Run f(lo, hi, block) over fixed blocks of `block` cells of [0, n) on
+// the codec threads.  The blocks do not depend on the thread count, so
+// per-block partial results combined in block order are reproducible.
+template <typename F>
+void par_blocks(long long n, long long block, F f)
+{
+    const long long nb = (n + block - 1) / block;
+    const int T = (int)std::min<long long>(codec_threads(), nb);
+    auto run = [&](int t) {
+        for (long long b = t; b < nb; b += T) f(b * block, std::min(n, (b + 1) * block), b);
+    };
+    if (T <= 1) {
+        run(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) th.emplace_back(run, t);
+    run(0);
+    for (auto &x : th) x.join();
+}
+
+constexpr long long CODEC_BLOCK = 1 << 18;
 
 inline float pow2_at_least(double v)
 {
@@ -43,20 +82,43 @@ void choose_quant(const Geo &g, const HostMask &hm, double pin, const T *const s
     const long long N = g.N;
     const int nxy = g.nx * g.ny;
     const double slope = g.periodic ? 0.0 : -pin / g.nz;
+    const long long nb = (N + CODEC_BLOCK - 1) / CODEC_BLOCK;
+    std::vector<double> bsum(nb, 0.0);
+    std::vector<long long> bcnt(nb, 0);
+    par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long b) {
+        double sum = 0;
+        long long n = 0;
+        for (long long i = lo; i < hi; i++) {
+            if (hm.wall(i)) continue;
+            sum += (double)src[0][i] - slope * (double)(i / nxy);
+            n++;
+        }
+        bsum[b] = sum;
+        bcnt[b] = n;
+    });
     double sum = 0;
     long long n = 0;
-    for (long long i = 0; i < N; i++) {
-        if (hm.wall(i)) continue;
-        sum += (double)src[0][i] - slope * (double)(i / nxy);
-        n++;
+    for (long long b = 0; b < nb; b++) {
+        sum += bsum[b];
+        n += bcnt[b];
     }
     const double base = n ? sum / (double)n : 0.0;
+    std::vector<double> bp(nb, 0.0), bu(nb, 0.0);
+    par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long b) {
+        double pmax = 0, umax = 0;
+        for (long long i = lo; i < hi; i++) {
+            if (hm.wall(i)) continue;
+            const double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
+            pmax = std::max(pmax, std::fabs(dp));
+            for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
+        }
+        bp[b] = pmax;
+        bu[b] = umax;
+    });
     double pmax = 0, umax = 0;
-    for (long long i = 0; i < N; i++) {
-        if (hm.wall(i)) continue;
-        const double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
-        pmax = std::max(pmax, std::fabs(dp));
-        for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
+    for (long long b = 0; b < nb; b++) {
+        pmax = std::max(pmax, bp[b]);
+        umax = std::max(umax, bu[b]);
     }
     // headroom for growth: at least 1e-3 for velocity, 1e-4 for pressure
     umax = std::max(umax, 1e-3);
@@ -82,13 +144,15 @@ void encode_state(const Geo &g, const HostMask &hm, int precision, const Quant &
         using S = typename ST::S;
         using R = typename ST::R;
         S *d = (S *)dst;
-        for (long long i = 0; i < N; i++) {
-            const bool wall = hm.wall(i);
-            const int z = (int)(i / nxy);
-            d[i] = wall ? S(0) : ST::st_p((R)src[0][i], z, q);
-            for (int f = 1; f < 4; f++)
-                d[f * N + i] = wall ? S(0) : ST::st_u((R)src[f][i], q);
-        }
+        par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long) {
+            for (long long i = lo; i < hi; i++) {
+                const bool wall = hm.wall(i);
+                const int z = (int)(i / nxy);
+                d[i] = wall ? S(0) : ST::st_p((R)src[0][i], z, q);
+                for (int f = 1; f < 4; f++)
+                    d[f * N + i] = wall ? S(0) : ST::st_u((R)src[f][i], q);
+            }
+        });
     };
     switch (precision) {
     case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
@@ -108,13 +172,15 @@ void decode_state(const Geo &g, const HostMask &hm, int precision, const Quant &
         using ST = decltype(tag);
         using S = typename ST::S;
         const S *s = (const S *)src;
-        for (long long i = 0; i < N; i++) {
-            const bool wall = hm.wall(i);
-            const int z = (int)(i / nxy);
-            dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, q);
-            for (int f = 1; f < 4; f++)
-                dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], q);
-        }
+        par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long) {
+            for (long long i = lo; i < hi; i++) {
+                const bool wall = hm.wall(i);
+                const int z = (int)(i / nxy);
+                dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, q);
+                for (int f = 1; f < 4; f++)
+                    dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], q);
+            }
+        });
     };
     switch (precision) {
     case FSLBM_F64: run(Store<FSLBM_F64>{}); break;
@@ -158,5 +224,39 @@ std::vector<T> wall_weights(int nx, int ny, const uint8_t *m2)
         }
     return om;
 }
+
+// A domain validated and encoded on the host, ready to be uploaded into a
+// batch (its precision mode's storage).
+struct PreparedDomain {
+    Geo g{};
+    fslbm_params prm{};
+    int precision = FSLBM_F32;
+    Quant q{1.f, 1.f, 1.f, 1.f, 0.f, 0.f};
+    std::vector<uint8_t> mask2d;      // one z plane; empty for a periodic box
+    std::vector<unsigned char> host;  // encoded state, 4 planes
+    std::vector<float> omega;         // wall weights (walled domains)
+    size_t bytes = 0;                 // device bytes in the batch
+};
+
+// A domain's stored state copied to the host, with what decoding it
+// needs: the geometry, the precision mode, its quantisation and its
+// z-invariant mask plane (empty for a periodic box).
+struct HostResult {
+    Geo g{};
+    int precision = FSLBM_F32;
+    Quant q{1.f, 1.f, 1.f, 1.f, 0.f, 0.f};
+    std::vector<uint8_t> mask2d;
+    std::vector<unsigned char> raw;
+    long steps = 0;
+
+    void decode(double *const dst[4]) const
+    {
+        HostMask hm;
+        hm.m = g.periodic ? nullptr : mask2d.data();
+        hm.nxy = (long long)g.nx * g.ny;
+        hm.plane = true;
+        decode_state(g, hm, precision, q, raw.data(), dst);
+    }
+};
 
 }  // namespace fslbm

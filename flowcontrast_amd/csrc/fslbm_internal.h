@@ -89,6 +89,19 @@ int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, in
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
                    int parity);
 void batch_set_parity(void *dst, int precision, int parity);
+struct HostResult;      // fslbm_codec.h
+struct PreparedDomain;  // fslbm_codec.h
+// Validate and encode a domain on the host (no device work), then upload
+// and admit it; batch_add_plane does both.
+int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                  int ny, int nz, const double *const src[4], PreparedDomain &pd);
+int batch_attach(fslbm_batch *b, PreparedDomain &pd);
+// Admit a domain whose mask (NULL: periodic box) is one z plane of nx*ny
+// codes; src holds the four initial fields.  Returns the slot or an error.
+int batch_add_plane(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                    int ny, int nz, const double *const src[4]);
+// Copy a batch domain's current state to the host, then remove it.
+int batch_take(fslbm_batch *b, int slot, HostResult &out);
 
 // Whole-field reductions (8 doubles, see fslbm_reduce) into dst (device).
 hipError_t launch_reduce(const Geo &g, int precision, const void *state,

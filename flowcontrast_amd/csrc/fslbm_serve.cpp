@@ -1,0 +1,379 @@
+// Serving loop: solve requests (pipes, periodic boxes) of different sizes,
+// run parameters, precision modes and step budgets are queued as they
+// arrive and run as batch domains.  One pump:
+//   1. launches up to `quantum` steps of every resident request (one
+//      batched launch sequence per precision mode, each mode on its own
+//      stream, so that they overlap);
+//   2. while those run, validates and encodes queued requests on the host,
+//      in arrival order, while the HBM budget holds them (memory of
+//      residents that finish in this pump counts as free);
+//   3. waits for the steps;
+//   4. hands finished requests back (state copied to the host, device
+//      memory freed);
+//   5. uploads the encoded requests, which step from the next pump on.
+// The pump's step count for a mode is the smallest remaining budget of
+// its residents, capped by the quantum, so no request runs past its
+// budget.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fslbm_codec.h"
+#include "fslbm_internal.h"
+
+using namespace fslbm;
+
+namespace {
+
+constexpr int NMODES = 3;  // FSLBM_F32, FSLBM_F16S, FSLBM_BF16S
+
+int mode_index(int precision)
+{
+    switch (precision) {
+    case FSLBM_F32: return 0;
+    case FSLBM_F16S: return 1;
+    case FSLBM_BF16S: return 2;
+    default: return -1;
+    }
+}
+
+enum class RState { Queued, Running, Done, Failed };
+
+struct Request {
+    fslbm_request r{};
+    std::vector<uint8_t> mask2d;  // one z plane; empty for a periodic box
+    std::vector<double> st[4];    // initial state until admission
+    int mode = 0;
+    int slot = -1;
+    long long left = 0;
+    size_t bytes = 0;
+    double t_submit = 0, t_admit = -1, t_done = -1;
+    RState state = RState::Queued;
+    int status = FSLBM_OK;
+    PreparedDomain prep;  // encoded state between preparation and upload
+    HostResult res;
+};
+
+thread_local std::string g_server_err;
+
+}  // namespace
+
+struct fslbm_server {
+    int d1 = 2;
+    int quantum = 100;
+    size_t budget = 0, used = 0;
+    fslbm_batch_t batch[NMODES] = {nullptr, nullptr, nullptr};
+    std::deque<long long> queue;        // arrival order
+    std::vector<long long> ready;       // encoded, waiting for upload
+    std::vector<long long> running;     // admitted, in admission order
+    std::map<long long, Request> reqs;  // every request not yet dropped
+    std::deque<long long> unreported;   // finished, not yet popped
+    long long next_id = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::string err;
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+int sfail(fslbm_server *s, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    (s ? s->err : g_server_err) = buf;
+    return code;
+}
+
+void free_state(Request &q)
+{
+    for (auto &v : q.st) std::vector<double>().swap(v);
+}
+
+int ensure_batch(fslbm_server *s, int mode)
+{
+    if (s->batch[mode]) return FSLBM_OK;
+    static const int prec[NMODES] = {FSLBM_F32, FSLBM_F16S, FSLBM_BF16S};
+    // the server enforces the shared budget; each batch may use all of it
+    int rc = fslbm_batch_create(prec[mode], s->d1, s->budget, &s->batch[mode]);
+    if (rc) return sfail(s, rc, "batch creation: %s", fslbm_batch_last_error(nullptr));
+    rc = fslbm_batch_use_graph(s->batch[mode], 1);
+    if (rc) return sfail(s, rc, "batch graph mode: %s", fslbm_batch_last_error(s->batch[mode]));
+    return FSLBM_OK;
+}
+
+void finish(fslbm_server *s, long long id, Request &q, RState st, int status, double t)
+{
+    q.state = st;
+    q.status = status;
+    q.t_done = t;
+    s->unreported.push_back(id);
+}
+
+// Encode queued requests in arrival order while the budget holds them
+// (host work, run while the pump's steps execute).  `freeing` is the
+// device memory of residents that finish in this pump: it is released
+// before any prepared request is uploaded.  Returns the number of
+// requests that failed validation, or an error.
+int prepare_queued(fslbm_server *s, size_t freeing)
+{
+    int failed = 0;
+    while (!s->queue.empty()) {
+        const long long id = s->queue.front();
+        Request &q = s->reqs.at(id);
+        if (s->used - freeing + q.bytes > s->budget) break;  // FIFO: no overtaking
+        int rc = ensure_batch(s, q.mode);
+        if (rc) return rc;
+        const double *src[4] = {q.st[0].data(), q.st[1].data(), q.st[2].data(), q.st[3].data()};
+        rc = batch_prepare(s->batch[q.mode], &q.r.params,
+                           q.mask2d.empty() ? nullptr : q.mask2d.data(), q.r.nx, q.r.ny,
+                           q.r.nz, src, q.prep);
+        s->queue.pop_front();
+        free_state(q);
+        if (rc < 0) {
+            // an invalid request fails alone; the loop goes on
+            finish(s, id, q, RState::Failed, rc, s->now());
+            failed++;
+            continue;
+        }
+        s->used += q.bytes;  // reserved until the request is handed back
+        s->ready.push_back(id);
+    }
+    return failed;
+}
+
+// Upload the prepared requests and make them resident.
+int attach_ready(fslbm_server *s)
+{
+    int failed = 0;
+    for (long long id : s->ready) {
+        Request &q = s->reqs.at(id);
+        const int rc = batch_attach(s->batch[q.mode], q.prep);
+        q.prep = PreparedDomain();
+        if (rc < 0) {
+            s->used -= q.bytes;
+            finish(s, id, q, RState::Failed, rc, s->now());
+            failed++;
+            continue;
+        }
+        q.slot = rc;
+        q.state = RState::Running;
+        q.t_admit = s->now();
+        s->running.push_back(id);
+    }
+    s->ready.clear();
+    return failed;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *out)
+{
+    g_server_err.clear();
+    if (!out) return FSLBM_EINVAL;
+    *out = nullptr;
+    if (d1 != 1 && d1 != 2) return sfail(nullptr, FSLBM_EINVAL, "d1 must be 1 or 2");
+    if (quantum < 2) return sfail(nullptr, FSLBM_EINVAL, "quantum must be >= 2");
+    if (hbm_budget == 0) return sfail(nullptr, FSLBM_EINVAL, "zero HBM budget");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return sfail(nullptr, FSLBM_ENODEV, "no HIP device visible");
+    fslbm_server *s = new fslbm_server();
+    s->d1 = d1;
+    s->quantum = quantum;
+    s->budget = hbm_budget;
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return FSLBM_OK;
+}
+
+void fslbm_server_destroy(fslbm_server_t s)
+{
+    if (!s) return;
+    for (auto &b : s->batch)
+        if (b) fslbm_batch_destroy(b);
+    delete s;
+}
+
+const char *fslbm_server_last_error(fslbm_server_t s)
+{
+    return s ? s->err.c_str() : g_server_err.c_str();
+}
+
+double fslbm_server_clock(fslbm_server_t s) { return s ? s->now() : 0.0; }
+
+long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req, const uint8_t *mask,
+                              const double *p, const double *ux, const double *uy,
+                              const double *uz)
+{
+    if (!s) return FSLBM_EINVAL;
+    if (!req || !p || !ux || !uy || !uz) return sfail(s, FSLBM_EINVAL, "NULL argument");
+    const int mode = mode_index(req->precision);
+    if (mode < 0)
+        return sfail(s, FSLBM_EINVAL, "requests run in FP32 or 16-bit shifted storage");
+    if (req->nsteps < 1) return sfail(s, FSLBM_EINVAL, "nsteps must be >= 1");
+    if (req->params.d1 != s->d1)
+        return sfail(s, FSLBM_EINVAL, "request d1 %d differs from the server's %d",
+                     req->params.d1, s->d1);
+    if (req->nx < 1 || req->ny < 1 || req->nz < 1) return sfail(s, FSLBM_EINVAL, "bad box");
+    const long long N = (long long)req->nx * req->ny * req->nz;
+    const long long nxy = (long long)req->nx * req->ny;
+    if (N * 4 >= (1ll << 31)) return sfail(s, FSLBM_EINVAL, "request too large");
+    if (mask)
+        for (int z = 1; z < req->nz; z++)
+            if (std::memcmp(mask, mask + z * nxy, nxy) != 0)
+                return sfail(s, FSLBM_EINVAL, "request masks must be the same in every z plane");
+    const long long bytes = fslbm_batch_domain_bytes(req->precision, mask, req->nx, req->ny, req->nz);
+    if (bytes < 0) return sfail(s, (int)bytes, "bad box");
+    if ((size_t)bytes > s->budget)
+        return sfail(s, FSLBM_EINVAL, "request needs %lld device bytes, more than the budget %zu",
+                     bytes, s->budget);
+    const long long id = s->next_id++;
+    Request &q = s->reqs[id];
+    q.r = *req;
+    q.mode = mode;
+    q.left = req->nsteps;
+    q.bytes = (size_t)bytes;
+    if (mask) q.mask2d.assign(mask, mask + nxy);
+    const double *src[4] = {p, ux, uy, uz};
+    for (int f = 0; f < 4; f++) q.st[f].assign(src[f], src[f] + N);
+    q.t_submit = s->now();
+    s->queue.push_back(id);
+    return id;
+}
+
+int fslbm_server_pump(fslbm_server_t s)
+{
+    if (!s) return FSLBM_EINVAL;
+    // 1. launch the steps of the residents (asynchronous, one stream per mode)
+    long long qs[NMODES] = {0, 0, 0};
+    for (long long id : s->running) {
+        const Request &q = s->reqs.at(id);
+        long long &m = qs[q.mode];
+        m = m == 0 ? q.left : std::min(m, q.left);
+    }
+    size_t freeing = 0;
+    for (int k = 0; k < NMODES; k++) qs[k] = std::min<long long>(qs[k], s->quantum);
+    for (long long id : s->running) {
+        const Request &q = s->reqs.at(id);
+        if (q.left == qs[q.mode]) freeing += q.bytes;
+    }
+    int rc;
+    for (int k = 0; k < NMODES; k++)
+        if (qs[k] > 0) {
+            rc = fslbm_batch_step(s->batch[k], (int)qs[k]);
+            if (rc) return sfail(s, rc, "batch step: %s", fslbm_batch_last_error(s->batch[k]));
+        }
+    // 2. meanwhile encode the queued requests the budget admits
+    rc = prepare_queued(s, freeing);
+    if (rc < 0) return rc;
+    int finished = rc;
+    // 3. wait for the steps
+    for (int k = 0; k < NMODES; k++)
+        if (qs[k] > 0) {
+            rc = fslbm_batch_sync(s->batch[k]);
+            if (rc) return sfail(s, rc, "batch sync: %s", fslbm_batch_last_error(s->batch[k]));
+        }
+    // 4. hand back what finished (admission order), freeing its memory
+    std::vector<long long> still;
+    still.reserve(s->running.size());
+    for (long long id : s->running) {
+        Request &q = s->reqs.at(id);
+        q.left -= qs[q.mode];
+        if (q.left > 0) {
+            still.push_back(id);
+            continue;
+        }
+        rc = batch_take(s->batch[q.mode], q.slot, q.res);
+        if (rc) return sfail(s, rc, "result copy: %s", fslbm_batch_last_error(s->batch[q.mode]));
+        s->used -= q.bytes;
+        q.slot = -1;
+        finish(s, id, q, RState::Done, FSLBM_OK, s->now());
+        finished++;
+    }
+    s->running.swap(still);
+    // 5. upload the prepared requests: they step from the next pump on
+    rc = attach_ready(s);
+    if (rc < 0) return rc;
+    return finished + rc;
+}
+
+int fslbm_server_next_done(fslbm_server_t s, fslbm_completion *out)
+{
+    if (!s || !out) return FSLBM_EINVAL;
+    // a request dropped before it was reported is skipped
+    while (!s->unreported.empty() && !s->reqs.count(s->unreported.front()))
+        s->unreported.pop_front();
+    if (s->unreported.empty()) return 0;
+    const long long id = s->unreported.front();
+    s->unreported.pop_front();
+    const Request &q = s->reqs.at(id);
+    out->id = id;
+    out->status = q.status;
+    out->steps = q.state == RState::Done ? q.res.steps : 0;
+    out->t_submit = q.t_submit;
+    out->t_admit = q.t_admit;
+    out->t_done = q.t_done;
+    return 1;
+}
+
+int fslbm_server_fetch(fslbm_server_t s, long long id, double *p, double *ux, double *uy,
+                       double *uz)
+{
+    if (!s) return FSLBM_EINVAL;
+    if (!p || !ux || !uy || !uz) return sfail(s, FSLBM_EINVAL, "NULL state array");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return sfail(s, FSLBM_EINVAL, "no request %lld", id);
+    const Request &q = it->second;
+    if (q.state == RState::Failed)
+        return sfail(s, FSLBM_ESTATE, "request %lld failed (%d)", id, q.status);
+    if (q.state != RState::Done) return sfail(s, FSLBM_ESTATE, "request %lld has not finished", id);
+    double *dst[4] = {p, ux, uy, uz};
+    q.res.decode(dst);
+    return FSLBM_OK;
+}
+
+int fslbm_server_drop(fslbm_server_t s, long long id)
+{
+    if (!s) return FSLBM_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return sfail(s, FSLBM_EINVAL, "no request %lld", id);
+    if (it->second.state != RState::Done && it->second.state != RState::Failed)
+        return sfail(s, FSLBM_ESTATE, "request %lld has not finished", id);
+    s->reqs.erase(it);
+    return FSLBM_OK;
+}
+
+int fslbm_server_query(fslbm_server_t s, int *queued, int *running, int *finished, size_t *used,
+                       size_t *budget)
+{
+    if (!s) return FSLBM_EINVAL;
+    int fin = 0;
+    for (const auto &kv : s->reqs)
+        if (kv.second.state == RState::Done || kv.second.state == RState::Failed) fin++;
+    if (queued) *queued = (int)(s->queue.size() + s->ready.size());
+    if (running) *running = (int)s->running.size();
+    if (finished) *finished = fin;
+    if (used) *used = s->used;
+    if (budget) *budget = s->budget;
+    return FSLBM_OK;
+}
+
+}  // extern "C"

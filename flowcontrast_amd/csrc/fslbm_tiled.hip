@@ -684,10 +684,15 @@ __device__ __forceinline__ void pipe_item(
         }
         // Lanes that own no fluid cell store 0 to their (wall) column: in
         // a walled box an outside lane's clamped column is a wall.  In a
-        // periodic box an outside lane is a copy of an inside lane and
-        // stores the identical value.
+        // periodic box an outside lane computes a wrapped copy of an inside
+        // cell, but along another path (its neighbours come from other
+        // lanes and waves), so its result may differ in the last bit: its
+        // store offset is moved past the end of the buffer, where the
+        // buffer range check drops the write.  (offset + plane offset
+        // stays below 2^32: fields are below 2^31 bytes.)
         const bool store = PERIODIC ? true : (inside && fluid);
         const unsigned cb_ = col * (unsigned)sizeof(S);
+        const unsigned sb_ = (PERIODIC && !inside) ? 0x80000000u : cb_;
         const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oU = mkbuf(out + N, fbytes),
                                      oV = mkbuf(out + 2 * N, fbytes), oW = mkbuf(out + 3 * N, fbytes);
         // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
@@ -773,10 +778,10 @@ __device__ __forceinline__ void pipe_item(
                 const R np_ = (Ps * inv216 + w0) - P.gauge;
                 const unsigned so = poff(c);
                 const S z0 = S(0);
-                bstore(store ? ST::st_p(np_, c, P.q) : z0, oP, cb_, so);
-                bstore(store ? ST::st_u(nx_, P.q) : z0, oU, cb_, so);
-                bstore(store ? ST::st_u(ny_, P.q) : z0, oV, cb_, so);
-                bstore(store ? ST::st_u(nz_, P.q) : z0, oW, cb_, so);
+                bstore(store ? ST::st_p(np_, c, P.q) : z0, oP, sb_, so);
+                bstore(store ? ST::st_u(nx_, P.q) : z0, oU, sb_, so);
+                bstore(store ? ST::st_u(ny_, P.q) : z0, oV, sb_, so);
+                bstore(store ? ST::st_u(nz_, P.q) : z0, oW, sb_, so);
             }
             if (doB) {  // stage B: plane a-1; corrector tile of a-1
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);

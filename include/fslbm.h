@@ -282,6 +282,68 @@ int fslbm_batch_autotune(fslbm_batch_t b, int trial_steps, int *best_chunk,
 int fslbm_batch_time_steps(fslbm_batch_t b, int nsteps, double *ms);
 int fslbm_batch_sync(fslbm_batch_t b);
 
+/* ------------------------------------------------------------------ */
+/* Serving loop (no counterpart in the reference): solve requests of    */
+/* different sizes, run parameters and step budgets arrive at any time. */
+/* Each pump advances every resident request by up to `quantum` steps   */
+/* in one batched launch sequence per precision mode (the modes overlap */
+/* on separate streams); meanwhile it encodes queued requests on the    */
+/* host in arrival order while the HBM budget holds them; then it hands */
+/* finished requests back (state copied to the host, device memory      */
+/* freed) and uploads the encoded ones, which join the residents at the */
+/* next pump (continuous batching).  A request's result is bitwise the  */
+/* same as a solo run of the tiled single-domain path in its precision. */
+/* ------------------------------------------------------------------ */
+typedef struct fslbm_server *fslbm_server_t;
+
+typedef struct fslbm_request {
+    fslbm_params params; /* d1 must equal the server's                    */
+    int nx, ny, nz;      /* box; the mask (if any) equal in every z plane */
+    int precision;       /* FSLBM_F32, FSLBM_F16S or FSLBM_BF16S          */
+    long long nsteps;    /* step budget of the solve (>= 1)               */
+} fslbm_request;
+
+typedef struct fslbm_completion {
+    long long id;        /* as returned by fslbm_server_submit            */
+    int status;          /* FSLBM_OK, or the error the request failed with */
+    long long steps;     /* steps run                                     */
+    double t_submit, t_admit, t_done; /* seconds on the server's clock    */
+} fslbm_completion;
+
+/* quantum: steps per pump at most (>= 2); hbm_budget: device bytes the
+ * resident requests may hold together. */
+int fslbm_server_create(int d1, size_t hbm_budget, int quantum,
+                        fslbm_server_t *out);
+void fslbm_server_destroy(fslbm_server_t s);
+const char *fslbm_server_last_error(fslbm_server_t s);
+/* Seconds since the server was created (the clock of the completions). */
+double fslbm_server_clock(fslbm_server_t s);
+
+/* Queue a request; the initial state (four arrays of nx*ny*nz doubles)
+ * and mask (NULL: periodic box) are copied.  Returns the request id
+ * (>= 0) or an error (a request larger than the whole budget is
+ * FSLBM_EINVAL). */
+long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req,
+                              const uint8_t *mask, const double *p,
+                              const double *ux, const double *uy,
+                              const double *uz);
+
+/* One scheduling round (blocks until its steps are done).  Returns the
+ * number of requests that finished in it (>= 0) or an error. */
+int fslbm_server_pump(fslbm_server_t s);
+
+/* Pop the oldest unreported completion: 1 if one was written, 0 if none. */
+int fslbm_server_next_done(fslbm_server_t s, fslbm_completion *out);
+
+/* A finished request's final state (decoded to doubles); the result is
+ * kept until fslbm_server_drop. */
+int fslbm_server_fetch(fslbm_server_t s, long long id, double *p, double *ux,
+                       double *uy, double *uz);
+int fslbm_server_drop(fslbm_server_t s, long long id);
+
+int fslbm_server_query(fslbm_server_t s, int *queued, int *running,
+                       int *finished, size_t *used, size_t *budget);
+
 #ifdef __cplusplus
 }
 #endif

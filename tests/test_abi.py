@@ -101,3 +101,18 @@ def test_laminar_and_blasius_pin():
     pin = inputs.laminar_pin(0.01, 64, 0.0714, 14.0)
     assert abs(pin * 196 / (24 * 0.01 * 64) - 0.0714) < 1e-12
     assert inputs.blasius_pin(0.01, 1024, 5300, 120) > 0
+
+
+def test_server_create_validates_without_gpu():
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.fslbm_server_create(3, 1 << 30, 100, C.byref(h)) == -1
+    assert b"d1" in L.fslbm_server_last_error(None)
+    assert L.fslbm_server_create(2, 1 << 30, 1, C.byref(h)) == -1
+    assert b"quantum" in L.fslbm_server_last_error(None)
+    assert L.fslbm_server_create(2, 0, 100, C.byref(h)) == -1
+    # a valid configuration needs a device: none here
+    rc = L.fslbm_server_create(2, 1 << 30, 100, C.byref(h))
+    assert rc in (0, -4)
+    if rc == 0:
+        L.fslbm_server_destroy(h)

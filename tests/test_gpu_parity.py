@@ -148,12 +148,18 @@ def test_taylor_green_periodic(orc, variant, precision, tol):
     assert max(err) <= tol, err
 
 
+ODD_CASES = [(37, 29, 23, 11.0, 1, 0.5),
+             (70, 66, 17, 30.0, 3, 1.0),
+             (9, 9, 8, 3.0, 2, 0.0),
+             (131, 77, 19, 37.0, 2, 1.0)]
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
-def test_odd_shapes_and_d1(orc, variant):
-    """Ragged sizes (not multiples of the tile), d1 = 1 and 3, blend."""
-    for (nx, ny, nz, R, d1, blend) in [(37, 29, 23, 11.0, 1, 0.5),
-                                       (70, 66, 17, 30.0, 3, 1.0),
-                                       (9, 9, 8, 3.0, 2, 0.0)]:
+@pytest.mark.parametrize("case", range(len(ODD_CASES)))
+def test_odd_shapes_and_d1(orc, variant, case):
+    """Ragged sizes (not multiples of the tile), d1 = 1, 2, 3, blend.  The
+    tiled kernel takes d1 <= 2 and nz >= 16; other cases skip for it."""
+    for (nx, ny, nz, R, d1, blend) in [ODD_CASES[case]]:
         mask, _ = inputs.circle_mask(nx, ny, nz, R)
         pin = 2e-3
         st = inputs.poiseuille_state(mask, R, 0.03, pin, 0.05, 5)
