@@ -32,6 +32,7 @@ struct fslbm_ctx {
     Quant q{1.f, 1.f, 1.f, 1.f, 0.f, 0.f};
     void *state[2] = {nullptr, nullptr};
     int cur = 0;
+    uint32_t steps = 0;        // steps since set_state (stochastic rounding)
     uint8_t *mask = nullptr;   // device mask (nullptr: periodic box)
     std::vector<uint8_t> hmask;
     bool z_invariant = false;  // every z plane of the mask is the same
@@ -132,6 +133,7 @@ int set_state_impl(fslbm_ctx *c, const T *p, const T *ux, const T *uy,
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "state upload failed: %s", hipGetErrorString(e));
     c->gauge = 0.0;
+    c->steps = 0;
     return FSLBM_OK;
 }
 
@@ -176,6 +178,7 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     a.blend = c->prm.blend;
     a.d1 = c->prm.d1;
     a.q = c->q;
+    a.seed = c->steps;
     const bool shifted = c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S;
     // Shifted storage applies the gauge to the pressure reference: the
     // stored deviations are unchanged and later loads see p - gauge.
@@ -193,6 +196,7 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "step launch failed: %s", hipGetErrorString(e));
     c->cur ^= 1;
+    c->steps++;
     if (shifted && gauge != 0.0) c->q.pbase = (float)((double)c->q.pbase - gauge);
     return FSLBM_OK;
 }

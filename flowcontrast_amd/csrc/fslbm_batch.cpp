@@ -49,6 +49,8 @@ struct fslbm_batch {
     std::vector<Domain> doms;
     void *d_doms = nullptr;
     size_t d_doms_cap = 0;
+    uint32_t *d_seeds = nullptr;  // two step counters per domain slot
+    size_t d_seeds_cap = 0;
     int2 *d_work[NCLASS] = {nullptr, nullptr, nullptr, nullptr};
     size_t work_cap[NCLASS] = {0, 0, 0, 0};
     int nwork[NCLASS] = {0, 0, 0, 0};
@@ -98,6 +100,7 @@ StepArgs step_args(const Domain &d)
     a.d1 = d.prm.d1;
     a.gauge = 0.0;
     a.q = d.q;
+    a.seed = (uint32_t)d.steps;
     return a;
 }
 
@@ -113,6 +116,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     const size_t db = batch_dom_bytes(b->precision);
     const size_t nd = b->doms.size();
     std::vector<unsigned char> host(std::max<size_t>(nd, 1) * db, 0);
+    std::vector<uint32_t> seeds(2 * std::max<size_t>(nd, 1), 0u);
     std::vector<int2> work[NCLASS];
     for (size_t i = 0; i < nd; i++) {
         Domain &d = b->doms[i];
@@ -123,6 +127,8 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         const int nw = batch_fill_dom(host.data() + i * db, b->precision, d.g, step_args(d), ch,
                                       d.buf[0], d.buf[1], d.mask2d, d.omega, d.parity);
         if (nw < 0) return bfail(b, FSLBM_EINVAL, "domain %zu: unsupported precision", i);
+        seeds[2 * i] = (uint32_t)d.steps;  // the next launch has flip 0
+        seeds[2 * i + 1] = (uint32_t)d.steps + 1u;
         auto &wl = work[class_of(d)];
         for (int w = 0; w < nw; w++) wl.push_back(make_int2((int)i, w));
     }
@@ -135,6 +141,15 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         b->d_doms_cap = host.size();
     }
     BCHK(b, hipMemcpyAsync(b->d_doms, host.data(), host.size(), hipMemcpyHostToDevice, b->stream));
+    const size_t sb = seeds.size() * sizeof(uint32_t);
+    if (sb > b->d_seeds_cap) {
+        if (b->d_seeds) (void)hipFree(b->d_seeds);
+        b->d_seeds = nullptr;
+        b->d_seeds_cap = 0;
+        BCHK(b, hipMalloc((void **)&b->d_seeds, sb));
+        b->d_seeds_cap = sb;
+    }
+    BCHK(b, hipMemcpyAsync(b->d_seeds, seeds.data(), sb, hipMemcpyHostToDevice, b->stream));
     for (int c = 0; c < NCLASS; c++) {
         const size_t need = std::max<size_t>(work[c].size(), 1) * sizeof(int2);
         if (need > b->work_cap[c]) {
@@ -162,6 +177,7 @@ hipError_t launch_step_all(fslbm_batch *b, int flip)
         if (b->nwork[c] == 0) continue;
         BatchLaunch L;
         L.doms = b->d_doms;
+        L.seeds = b->d_seeds;
         L.work = b->d_work[c];
         L.nwork = b->nwork[c];
         L.flip = flip;
@@ -408,6 +424,7 @@ void fslbm_batch_destroy(fslbm_batch_t b)
         if (d.omega) (void)hipFree(d.omega);
     }
     if (b->d_doms) (void)hipFree(b->d_doms);
+    if (b->d_seeds) (void)hipFree(b->d_seeds);
     for (auto *w : b->d_work) if (w) (void)hipFree(w);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);

@@ -120,9 +120,14 @@ void choose_quant(const Geo &g, const HostMask &hm, double pin, const T *const s
         pmax = std::max(pmax, bp[b]);
         umax = std::max(umax, bu[b]);
     }
-    // headroom for growth: at least 1e-3 for velocity, 1e-4 for pressure
+    // Floors leave room for growth from a quiet start (the fp16 range
+    // reaches 64x past the scaled maximum of 1024, and small values keep
+    // their 11 significant bits): 1e-3 for velocity; for pressure the
+    // acoustic bound of a velocity perturbation, |dp| <= sqrt(3)|du| <
+    // 2 umax, and a tenth of the jump.  The jump itself is carried by the
+    // reference line pbase + pslope*z.
     umax = std::max(umax, 1e-3);
-    pmax = std::max(pmax, std::max(1e-4, std::fabs(pin)));
+    pmax = std::max(pmax, std::max(2.0 * umax, 0.1 * std::fabs(pin)));
     const float su = pow2_at_least(umax) / 1024.0f;
     const float sp = pow2_at_least(pmax) / 1024.0f;
     q.su = su;

@@ -33,6 +33,7 @@ struct StepK {
     R c0, c1, c2;
     R pin, gauge;
     int d1;
+    uint32_t skey;    // stochastic-rounding key of this step (16-bit modes)
 };
 
 // Host-side description of a step launch.
@@ -40,6 +41,7 @@ struct StepArgs {
     double tblend, pin, nu, blend, gauge;
     int d1;
     Quant q;
+    uint32_t seed = 0;  // step counter of the domain (stochastic rounding)
 };
 
 // Build the per-mode constants: exact rationals for FSLBM_F64, the
@@ -76,6 +78,7 @@ int default_chunk(const Geo &g, int precision);
 // descriptors' parities were written, mod 2.
 struct BatchLaunch {
     const void *doms;
+    uint32_t *seeds;  // two step counters per domain (see k_step_batch)
     const int2 *work;
     int nwork, flip, grid;
     bool periodic, fd2;

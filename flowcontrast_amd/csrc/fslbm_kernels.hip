@@ -39,6 +39,7 @@ StepK<double> make_consts_f64(const StepArgs &a)
     k.pin = a.pin;
     k.gauge = a.gauge;
     k.d1 = a.d1;
+    k.skey = sr_key(a.seed);
     return k;
 }
 
@@ -62,6 +63,7 @@ StepK<float> make_consts_f32(const StepArgs &a)
     k.pin = (float)a.pin;
     k.gauge = (float)a.gauge;
     k.d1 = a.d1;
+    k.skey = sr_key(a.seed);
     return k;
 }
 
@@ -171,10 +173,11 @@ __global__ __launch_bounds__(256) void k_step_plain(
 
     const R g = k.dtddy * k.m;
     auto *Po = out, *Uo = out + N, *Vo = out + 2 * N, *Wo = out + 3 * N;
-    Po[i] = ST::st_p(rt - k.gauge, z, q);
-    Uo[i] = ST::st_u((ax + g * Lx) * k.c + us * k.cc, q);
-    Vo[i] = ST::st_u((ay + g * Ly) * k.c + vs * k.cc, q);
-    Wo[i] = ST::st_u((az + g * Lz) * k.c + ws * k.cc, q);
+    const SrWords rw = ST::SR ? sr_words((uint32_t)i, k.skey) : SrWords{0u, 0u};
+    Po[i] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
+    Uo[i] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
+    Vo[i] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);
+    Wo[i] = ST::st_u_sr((az + g * Lz) * k.c + ws * k.cc, q, rw.b << 16);
 }
 
 template <int MODE>

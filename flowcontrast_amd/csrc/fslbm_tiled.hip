@@ -157,6 +157,7 @@ struct TiledParams {
     // unnormalised passes: u' = ku*u + ks1*S1 + ks2*S2 + kx*X + tb*omega*p
     R ku, ks1, ks2, kx;
     Quant q;
+    uint32_t skey;  // stochastic-rounding key of this step (16-bit modes)
 };
 
 __device__ __forceinline__ int wrapi(int v, int n)
@@ -557,7 +558,7 @@ __device__ __forceinline__ void pipe_item(
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
     const typename Store<MODE>::R *__restrict__ omega,
-    const TiledParams<typename Store<MODE>::R> &P, const int w)
+    const TiledParams<typename Store<MODE>::R> &P, const int w, const uint32_t skey)
 {
     using ST = Store<MODE>;
     using R = typename ST::R;
@@ -778,10 +779,12 @@ __device__ __forceinline__ void pipe_item(
                 const R np_ = (Ps * inv216 + w0) - P.gauge;
                 const unsigned so = poff(c);
                 const S z0 = S(0);
-                bstore(store ? ST::st_p(np_, c, P.q) : z0, oP, sb_, so);
-                bstore(store ? ST::st_u(nx_, P.q) : z0, oU, sb_, so);
-                bstore(store ? ST::st_u(ny_, P.q) : z0, oV, sb_, so);
-                bstore(store ? ST::st_u(nz_, P.q) : z0, oW, sb_, so);
+                const SrWords rw = ST::SR ? sr_words((uint32_t)c * (uint32_t)nxy + col, skey)
+                                          : SrWords{0u, 0u};
+                bstore(store ? ST::st_p_sr(np_, c, P.q, rw.a) : z0, oP, sb_, so);
+                bstore(store ? ST::st_u_sr(nx_, P.q, rw.a << 16) : z0, oU, sb_, so);
+                bstore(store ? ST::st_u_sr(ny_, P.q, rw.b) : z0, oV, sb_, so);
+                bstore(store ? ST::st_u_sr(nz_, P.q, rw.b << 16) : z0, oW, sb_, so);
             }
             if (doB) {  // stage B: plane a-1; corrector tile of a-1
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
@@ -1013,7 +1016,7 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const int bid = blockIdx.x;
     const int w = (bid & 7) * P.per_xcd + (bid >> 3);
     if (w >= P.nwork) return;
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, w);
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, w, P.skey);
 }
 
 // ---------------------------------------------------------------------
@@ -1032,10 +1035,15 @@ struct BatchDom {
     int pad_;
 };
 
+// Step counters (stochastic rounding) live beside the read-only
+// descriptors, two per domain: the launch with flip f reads seeds[2d + f];
+// the workgroup of the domain's first work item writes seeds[2d + (f ^ 1)]
+// = that + 1 for the next launch (no launch reads and writes the same
+// slot), so hipGraph replays of the two launches keep counting.
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
 __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const BatchDom<typename Store<MODE>::R> *__restrict__ doms,
-    const int2 *__restrict__ work, int nwork, int flip)
+    uint32_t *__restrict__ seeds, const int2 *__restrict__ work, int nwork, int flip)
 {
     using S = typename Store<MODE>::S;
     // XCD-major: XCD x owns items [x*per, (x+1)*per) and its blocks
@@ -1049,8 +1057,10 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
         const int2 wd = work[w];
         const BatchDom<typename Store<MODE>::R> &D = doms[wd.x];
         const int par = D.parity ^ flip;
+        const uint32_t seed = seeds[2 * wd.x + flip];
+        if (wd.y == 0 && threadIdx.x == 0) seeds[2 * wd.x + (flip ^ 1)] = seed + 1u;
         pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
-                                               D.mask2d, D.omega, D.P, wd.y);
+                                               D.mask2d, D.omega, D.P, wd.y, sr_key(seed));
     }
 }
 
@@ -1062,7 +1072,7 @@ hipError_t launch_batch_mode(const BatchLaunch &b)
     const auto *doms = (const BatchDom<R> *)b.doms;
     const dim3 grid(b.grid), block((TYP + 3) * 64);
 #define FSLBM_BATCH(PER, DD, F2)                                                        \
-    k_step_batch<MODE, PER, DD, F2, TYP><<<grid, block, 0, b.stream>>>(doms, b.work, \
+    k_step_batch<MODE, PER, DD, F2, TYP><<<grid, block, 0, b.stream>>>(doms, b.seeds, b.work, \
                                                                         b.nwork, b.flip)
     if (b.periodic) {
         if (b.d1 == 1) { if (b.fd2) FSLBM_BATCH(true, 1, true); else FSLBM_BATCH(true, 1, false); }
@@ -1110,6 +1120,7 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.ks2 = P.tb * P.g * P.c2;
     P.kx = P.tb / R(216);
     P.q = a.q;
+    P.skey = k.skey;
 }
 
 template <int MODE, bool PERIODIC, int D1>

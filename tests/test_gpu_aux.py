@@ -244,3 +244,32 @@ def test_autotune_keeps_state_and_results():
             outs.append(s.get_state(np.float32))
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["f16s", "bf16s"])
+@pytest.mark.parametrize("variant", _variants())
+def test_shifted_storage_rounding_is_unbiased(precision, variant):
+    """A slowly decaying shear wave in 16-bit storage: its per-step change
+    (about 5e-6) is below half a storage ulp of the amplitude 0.05, which
+    round-to-nearest would lose every step (the wave would not decay).
+    Stochastic rounding keeps the expected decay: after 1000 steps the
+    projection on sin(kz) matches lambda(k)^1000 of the closed form."""
+    n = 64
+    k = 2 * math.pi / n
+    tb, nu, b, d1, A = 1.0, 0.01, 0.0, 2, 0.05
+    prm = RunParams(tblend=tb, pin=0.0, nu=nu, d1=d1, delta=1, blend=b)
+    z = np.arange(n, dtype=np.float64)
+    prof = (A * np.sin(k * z))[:, None, None] * np.ones((1, n, n))
+    p = np.full((n, n, n), 1.0)
+    zero = np.zeros((n, n, n))
+    with Solver((n, n, n), prm, mask=None, precision=precision) as s:
+        s.set_kernel(variant)
+        s.set_state(p, prof, zero, zero)
+        s.step(1000)
+        got = s.get_state()
+    c1, c2 = 1 + b / 3, -b / 12
+    lam = (1 - tb) + tb * ((2 + math.cos(k)) / 3 + (nu - 1 / 6) / d1 ** 2 *
+                           (-2 * c1 * (1 - math.cos(k * d1)) - 2 * c2 * (1 - math.cos(2 * k * d1))))
+    amp = float(np.sum(got[1] * prof) / np.sum(prof * prof))
+    assert 1 - lam ** 1000 > 0.08  # the decay to be resolved
+    assert abs(amp - lam ** 1000) < 2e-3 * lam ** 1000, (amp, lam ** 1000)
