@@ -259,6 +259,91 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
     return out
 
 
+def radial_stats(mom, samples, nz, mask2d, R):
+    """Radial profiles (unit-width bins of r around the pipe axis) of the
+    mean axial velocity and the Reynolds stresses u'u', v'v', w'w' and the
+    radial-axial shear u_r'w', from per-(x,y) moment sums over z and
+    samples (p, u, v, w, pp, uu, vv, ww, uv, uw, vw)."""
+    ny, nx = mask2d.shape
+    n = float(samples * nz)
+    m = mom / n
+    mu, mv, mw = m[1], m[2], m[3]
+    uu = m[5] - mu * mu
+    vv = m[6] - mv * mv
+    ww = m[7] - mw * mw
+    uv = m[8] - mu * mv
+    uw = m[9] - mu * mw
+    vw = m[10] - mv * mw
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    dx, dy = xx - (nx - 1) / 2.0, yy - (ny - 1) / 2.0
+    r = np.hypot(dx, dy)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cx, cy = np.where(r > 0, dx / r, 0.0), np.where(r > 0, dy / r, 0.0)
+    urw = cx * uw + cy * vw
+    fluid = mask2d == 0
+    bins = np.floor(r).astype(int)
+    nb = int(np.floor(R)) + 1
+    out = {}
+    for name, f in (("w", mw), ("uu", uu), ("vv", vv), ("ww", ww), ("urw", urw)):
+        prof = np.zeros(nb)
+        for b_ in range(nb):
+            sel = fluid & (bins == b_)
+            prof[b_] = f[sel].mean() if sel.any() else 0.0
+        out[name] = prof
+    return out
+
+
+def bench_stats(steps: int, window: int, every: int, precisions):
+    """Long run of the turbulent-pipe workload in each precision; mean
+    axial profile and Reynolds stresses averaged over the last `window`
+    steps (a sample every `every` steps), compared with the FP64 run."""
+    from flowcontrast_amd import KERNEL_AUTO, RunParams, Solver
+    nx = ny = 256
+    nz, R, re, ubar = 1024, 120.0, 5300.0, 0.05
+    nu = ubar * 2 * R / re
+    mask, st, pin, nf = build_pipe(nx, ny, nz, R, nu, re, 0.05, 3)
+    prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=0.0)
+    prof = {}
+    for prec in precisions:
+        t0 = time.perf_counter()
+        with Solver(mask.shape, prm, mask=mask, precision=prec, kernel=KERNEL_AUTO) as s:
+            s.set_state(*st)
+            s.moments_reset()
+            done = 0
+            while done < steps:
+                n = min(every, steps - done)
+                s.step(n)
+                done += n
+                if done > steps - window:
+                    s.moments_accumulate()
+                if done % 5000 == 0:
+                    s.sync()
+                    print(f"bench: stats {prec} step {done} ({time.perf_counter() - t0:.0f} s)",
+                          file=sys.stderr, flush=True)
+            mom, samples = s.moments_read()
+            red = s.reduce()
+        prof[prec] = radial_stats(mom, samples, nz, mask[0], R)
+        prof[prec]["finite"] = bool(red[7] == 0)
+    ref = prof["f64"]
+    out = {"workload": f"pipe{nx}x{ny}x{nz}_r{int(R)}, Re {re:g}, 5% noise, {steps} steps, "
+                       f"moments over the last {window} (every {every})",
+           "steps": steps, "window": window, "every": every, "errors_vs_f64": {}}
+    wmax = float(np.abs(ref["w"]).max())
+    for prec in precisions:
+        if prec == "f64":
+            continue
+        e = {}
+        for name in ("w", "uu", "vv", "ww", "urw"):
+            a, b_ = prof[prec][name], ref[name]
+            scale = wmax if name == "w" else float(np.abs(b_).max())
+            e[name] = float(f"{float(np.abs(a - b_).max()) / scale:.3e}") if scale > 0 else None
+        e["finite"] = prof[prec]["finite"]
+        out["errors_vs_f64"][prec] = e
+    out["f64_profile_w_axis_wall"] = [float(ref["w"][0]), float(ref["w"][-1])]
+    out["f64_max_stress"] = {k: float(np.abs(ref[k]).max()) for k in ("uu", "vv", "ww", "urw")}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +359,8 @@ def main() -> int:
                          "default, -1 = autotune)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["all", "pipe", "batch", "serve"],
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "pipe", "batch", "serve", "stats"],
                     help="all: the pipe step line with the batched and serving "
                          "results attached")
     ap.add_argument("--batch-count", type=int, default=64)
@@ -285,6 +371,10 @@ def main() -> int:
     ap.add_argument("--serve-budget", type=int, default=32 << 30)
     ap.add_argument("--spot-checks", type=int, default=3)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--stats-steps", type=int, default=50000)
+    ap.add_argument("--stats-window", type=int, default=20000)
+    ap.add_argument("--stats-every", type=int, default=100)
+    ap.add_argument("--stats-precisions", default="f64,f32,f16s")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB x 10 (the "
                          "scale for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -304,6 +394,10 @@ def main() -> int:
     if args.workload == "batch":
         print(json.dumps(bench_batch(args.batch_count, args.batch_steps,
                                      not args.no_sequential)), flush=True)
+        return 0
+    if args.workload == "stats":
+        print(json.dumps(bench_stats(args.stats_steps, args.stats_window, args.stats_every,
+                                     args.stats_precisions.split(","))), flush=True)
         return 0
     if args.workload == "serve":
         print(json.dumps(bench_serve(args.serve_requests, args.serve_rate, args.serve_quantum,

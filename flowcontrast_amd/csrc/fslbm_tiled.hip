@@ -528,9 +528,6 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
 // Queue length 12 for both supported d1; 12 is a multiple of the 3-plane
 // message queue, the 4-plane halo prefetch and the 3-plane arm prefetch,
 // so every slot index is static in a 12-fold unrolled loop.
-template <int D1> struct PipeAhead;
-template <> struct PipeAhead<1> { static constexpr int LA = 5; };
-template <> struct PipeAhead<2> { static constexpr int LA = 1; };
 
 // x messages used from the right neighbour (lane+1) and the left one
 // (lane-1) in the x pass, and their order in the LDS edge blocks.
@@ -564,10 +561,14 @@ __device__ __forceinline__ void pipe_item(
     using R = typename ST::R;
     using S = typename ST::S;
     static_assert(sizeof(R) == 4, "pipelined kernel takes 32-bit arithmetic");
-    constexpr int HF = 2 * D1;
-    constexpr int LA = PipeAhead<D1>::LA;
-    constexpr int QL = 2 * HF + LA + 3;
-    static_assert(QL == 12, "queue length must be 12");
+    // Corrector reach: d1, and 2*d1 when the order-4 arms carry weight
+    // (blend != 0).  The z queue holds the planes c-HF .. a+HF+LA of the
+    // stage-C plane c = a-2; its 12 slots fix the look-ahead LA of the
+    // loads (1 plane for reach 4, 5 for reach 2, 7 for reach 1).
+    constexpr int HF = FD2 ? 2 * D1 : D1;
+    constexpr int QL = 12;
+    constexpr int LA = QL - 2 * HF - 3;
+    static_assert(LA >= 1, "queue too short for the corrector reach");
     constexpr int U = 12;
     constexpr int PD = 4;  // halo column prefetch distance (planes)
     constexpr int AD = 3;  // corrector arm prefetch distance (planes)
@@ -608,14 +609,15 @@ __device__ __forceinline__ void pipe_item(
     };
     auto colx = [&](int v) { return PERIODIC ? modp(v, nx) : clampi(v, nx); };
     auto coly = [&](int v) { return PERIODIC ? modp(v, ny) : clampi(v, ny); };
+    // Wall cells hold zero velocity in every storage mode, so only their
+    // pressure (the decoded reference line and the jump) needs masking for
+    // the whole message to vanish.
     auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl) -> Msg<R> {
         R jump;
         const int zp = phys(zl, jump);
-        const R pp = ST::ld_p(rp, zp, P.q) + jump;
+        const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
         const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
-        Msg<R> m = make_msg<R>(pp, ux, uy, uz);
-        if (!fl) m = zero_msg<R>();
-        return m;
+        return make_msg<R>(pp, ux, uy, uz);
     };
     // x pass of a row wave: neighbours by DPP, edge lanes from LDS
     auto xpass = [&](int buf, int ry, int ix, const R (&xv)[10], const R (&cen)[6],
