@@ -181,7 +181,7 @@ hipError_t launch_step_all(fslbm_batch *b, int flip)
         L.work = b->d_work[c];
         L.nwork = b->nwork[c];
         L.flip = flip;
-        L.grid = std::min(b->grid, 8 * ((b->nwork[c] + 7) / 8));
+        L.grid = 8 * ((b->nwork[c] + 7) / 8);  // one workgroup per item
         L.periodic = (c & 2) != 0;
         L.fd2 = (c & 1) != 0;
         L.d1 = b->d1;

@@ -1022,10 +1022,11 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
 }
 
 // ---------------------------------------------------------------------
-// Batched step: many independent domains advanced by one persistent
-// launch over a packed table of work items (domain, tile x chunk).
-// Every item runs pipe_item, the same code as the single-domain kernel,
-// so a domain's result does not depend on what else is in the batch.
+// Batched step: many independent domains advanced by one launch over a
+// packed table of work items (domain, tile x chunk), one workgroup per
+// item, in XCD-major order.  Every item runs pipe_item, the same code as
+// the single-domain kernel, so a domain's result does not depend on what
+// else is in the batch.
 // ---------------------------------------------------------------------
 template <typename R>
 struct BatchDom {
@@ -1048,22 +1049,18 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     uint32_t *__restrict__ seeds, const int2 *__restrict__ work, int nwork, int flip)
 {
     using S = typename Store<MODE>::S;
-    // XCD-major: XCD x owns items [x*per, (x+1)*per) and its blocks
-    // stride over them.
-    const int gx = gridDim.x >> 3;
-    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    // XCD-major: XCD x takes items [x*per, (x+1)*per)
     const int per = (nwork + 7) >> 3;
-    for (int i = j; i < per; i += gx) {
-        const int w = xcd * per + i;
-        if (w >= nwork) break;
-        const int2 wd = work[w];
-        const BatchDom<typename Store<MODE>::R> &D = doms[wd.x];
-        const int par = D.parity ^ flip;
-        const uint32_t seed = seeds[2 * wd.x + flip];
-        if (wd.y == 0 && threadIdx.x == 0) seeds[2 * wd.x + (flip ^ 1)] = seed + 1u;
-        pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
-                                               D.mask2d, D.omega, D.P, wd.y, sr_key(seed));
-    }
+    const int bid = blockIdx.x;
+    const int w = (bid & 7) * per + (bid >> 3);
+    if (w >= nwork) return;
+    const int2 wd = work[w];
+    const BatchDom<typename Store<MODE>::R> &D = doms[wd.x];
+    const int par = D.parity ^ flip;
+    const uint32_t seed = seeds[2 * wd.x + flip];
+    if (wd.y == 0 && threadIdx.x == 0) seeds[2 * wd.x + (flip ^ 1)] = seed + 1u;
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
+                                           D.mask2d, D.omega, D.P, wd.y, sr_key(seed));
 }
 
 template <int MODE>
