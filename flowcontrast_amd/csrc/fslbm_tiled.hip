@@ -49,14 +49,12 @@ constexpr int NI = TX * TY;       // interior threads
 constexpr int NH = 2 * TX;        // halo-row threads
 constexpr int NS = 2 * RY;        // halo-column threads
 constexpr int NT = 704;           // 11 waves
-// Pipelined kernel: a workgroup of TYP + 2 waves owns TXO = 62 output
-// columns x TYP rows.  Each row wave spans 64 lanes, the output columns and
-// one column on each side, whose predictor messages its neighbours read by
-// DPP.  Tile rows: 14 for FP32 (16 waves fit the 128-register budget), 10
-// for the 16-bit storage modes (their decode needs more registers: 12
-// waves at three per SIMD).
-constexpr int TXO = TX - 2;
-constexpr int pipe_rows(int mode) { return mode == FSLBM_F32 ? 14 : 10; }
+// tile rows of the pipelined kernel: 12 for FP32 (15 waves fit the
+// 128-register budget without spills), 8 for the 16-bit storage modes
+// (their decode needs more registers).  Waves go to the four SIMDs round
+// robin; these counts keep the busiest SIMD's VALU load lowest (13 rows
+// or 9 rows put a fourth or third interior wave on one SIMD).
+constexpr int pipe_rows(int mode) { return mode == FSLBM_F32 ? 12 : 8; }
 
 template <typename R>
 struct Msg {
@@ -513,15 +511,16 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
 
 // ---------------------------------------------------------------------
 // Software-pipelined variant for the 32-bit-arithmetic modes (FP32 and the
-// 16-bit shifted-storage modes).  Tile 62 x TYP, (TYP + 2) waves of 64
-// lanes; lane l covers column x0 - 1 + l, so each row wave carries its own
-// x halo (the end lanes compute messages only, and do not store):
+// 16-bit shifted-storage modes).  Tile 64 x TYP, (TYP + 3) waves:
 //   waves 0 .. TYP-1   interior rows
-//   waves TYP, TYP+1   halo rows y0-1, y0+TYP (+ corrector y and x arms)
+//   waves TYP, TYP+1   halo rows y0-1, y0+TYP (+ corrector y arms)
+//   wave  TYP+2        halo columns x0-1, x0+64 (+ corrector x arms)
 // Each iteration t advances three planes and ends in one barrier:
 //   stage A, plane a = zs+t : z pass -> x messages kept in registers
 //   stage B, plane a-1      : x pass, neighbours x+-1 through DPP wave
-//                             shifts, y messages -> ymsg[(a-1)&1]
+//                             shifts (the two tile-edge lanes take the
+//                             halo-column messages from LDS), y messages
+//                             -> ymsg[(a-1)&1]
 //   stage C, plane a-2      : y pass + corrector + blend + store
 // LDS buffers are double-buffered by plane parity.  Global loads are
 // unconditional buffer loads at clamped or wrapped addresses, decoded
@@ -540,13 +539,13 @@ constexpr int XL[6] = {1, 3, 4, 6, 8, 9};
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i reads lane i+1
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i reads lane i-1
 
-// value of `src` from the neighbouring lane (0 at the wave's ends, whose
-// results are not used); the move can fold into the consuming add
+// value of `src` from the neighbouring lane; lanes without a neighbour in
+// the wave keep `edge`
 template <int CTRL>
-__device__ __forceinline__ float lane_shift(float src)
+__device__ __forceinline__ float lane_shift(float src, float edge)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-        0, __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, true));
+        __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
 }
 
 // One work item (tile x chunk) of the pipelined step.  Called by the
@@ -576,22 +575,19 @@ __device__ __forceinline__ void pipe_item(
     constexpr int PD = 4;  // halo column prefetch distance (planes)
     constexpr int AD = 3;  // corrector arm prefetch distance (planes)
     constexpr int RYP = TYP + 2;
-    // Lane l of a row wave holds column x0 - 1 + l.  fd column c holds
-    // x = x0 - 1 - HF + c: the lanes at HF .. HF+63 and HF - 1 corrector
-    // arm columns on each side beyond them (columns 0 and FX-1 are read
-    // only by the two end lanes, whose results are not stored).
     constexpr int FX = TX + 2 * HF, FY = TYP + 2 * HF;
-    constexpr int NXH = (HF - 1) * TYP;  // arm cells loaded per halo-row wave
-    static_assert(NXH <= 64, "x-arm cells of a halo-row wave exceed a wave");
+    constexpr int NXA = 2 * HF * TYP;          // x-arm cells
+    constexpr int XPL = (NXA + 63) / 64;       // x-arm cells per lane
 
     __shared__ R ymsg[2][6][RYP][TX];
     __shared__ FdCell<R> fd[2][FY][FX];
+    __shared__ R edge[2][RYP][2][8];  // [side 0: column x0-1 | 1: x0+64]
 
     const int ntile = P.ntx * P.nty;
     const int chunk = w / ntile;
     const int tile = w - chunk * ntile;
     const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
-    const int x0 = tx * TXO, y0 = ty * TYP;
+    const int x0 = tx * TX, y0 = ty * TYP;
     const int zs = chunk * P.chunk;
     const int ze = min(zs + P.chunk, P.nz);
     const int n = ze - zs;
@@ -625,14 +621,20 @@ __device__ __forceinline__ void pipe_item(
         const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
         return make_msg<R>(pp, ux, uy, uz);
     };
-    // x pass of a row wave: neighbours by DPP
+    // x pass of a row wave: neighbours by DPP, edge lanes from LDS
     auto xpass = [&](int buf, int ry, int ix, const R (&xv)[10], const R (&cen)[6],
                      R &b0, R &g0, R &d0) {
+        R le[6], re[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            le[j] = edge[buf][ry][0][j];
+            re[j] = edge[buf][ry][1][j];
+        }
         R e[6], wv[6];
 #pragma unroll
         for (int j = 0; j < 6; j++) {
-            e[j] = lane_shift<DPP_WAVE_SHL1>(xv[XR[j]]);
-            wv[j] = lane_shift<DPP_WAVE_SHR1>(xv[XL[j]]);
+            e[j] = lane_shift<DPP_WAVE_SHL1>(xv[XR[j]], re[j]);
+            wv[j] = lane_shift<DPP_WAVE_SHR1>(xv[XL[j]], le[j]);
         }
         // e[] = values 0,2,4,5,7,9 of x+1; wv[] = values 1,3,4,6,8,9 of x-1
         b0 = R(4) * cen[0] + (e[0] + wv[0]);
@@ -674,10 +676,9 @@ __device__ __forceinline__ void pipe_item(
     if (wave < TYP) {
         // ================= interior rows =================
         const int ix = lane, iy = wave;
-        const int x = x0 - 1 + ix, y = y0 + iy;
+        const int x = x0 + ix, y = y0 + iy;
         const int ry = iy + 1;
-        const bool inside = x >= 0 && x < nx && y < ny;
-        const bool own = ix >= 1 && ix <= TXO && inside;
+        const bool inside = x < nx && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
         R om0 = 0, omx = 0, omy = 0;
@@ -686,16 +687,17 @@ __device__ __forceinline__ void pipe_item(
             omx = omega[nxy + col];
             omy = omega[2 * nxy + col];
         }
-        // Only the 62 output lanes inside the box store; a wall cell stores
-        // 0.  The others (the two end lanes, lanes past the box: another
-        // workgroup owns their cells, or their wrapped or clamped images)
-        // get a store offset past the end of the buffer, where the buffer
-        // range check drops the write, so every store instruction runs
-        // unconditionally.  (offset + plane offset stays below 2^32: fields
-        // are below 2^31 bytes.)
-        const bool store = PERIODIC ? true : fluid;
+        // Lanes that own no fluid cell store 0 to their (wall) column: in
+        // a walled box an outside lane's clamped column is a wall.  In a
+        // periodic box an outside lane computes a wrapped copy of an inside
+        // cell, but along another path (its neighbours come from other
+        // lanes and waves), so its result may differ in the last bit: its
+        // store offset is moved past the end of the buffer, where the
+        // buffer range check drops the write.  (offset + plane offset
+        // stays below 2^32: fields are below 2^31 bytes.)
+        const bool store = PERIODIC ? true : (inside && fluid);
         const unsigned cb_ = col * (unsigned)sizeof(S);
-        const unsigned sb_ = own ? cb_ : 0x80000000u;
+        const unsigned sb_ = (PERIODIC && !inside) ? 0x80000000u : cb_;
         const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oU = mkbuf(out + N, fbytes),
                                      oV = mkbuf(out + 2 * N, fbytes), oW = mkbuf(out + 3 * N, fbytes);
         // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
@@ -815,10 +817,10 @@ __device__ __forceinline__ void pipe_item(
         // ================= halo rows y0-1 and y0+TYP =================
         const int hr = wave - TYP;
         const int ix = lane;
-        const int x = x0 - 1 + ix;
+        const int x = x0 + ix;
         const int y = hr ? y0 + TYP : y0 - 1;
         const int ry = hr ? TYP + 1 : 0;
-        const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
+        const bool inside = x < nx && y >= 0 && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
         const unsigned cb_ = col * (unsigned)sizeof(S);
@@ -838,26 +840,9 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int r = 0; r < HF; r++) {
             const int fy = hr ? TYP + HF + r : r;
-            acol[r] = (colx(x) + nx * coly(y0 - HF + fy)) * (unsigned)sizeof(S);
+            acol[r] = (colx(x0 + ix) + nx * coly(y0 - HF + fy)) * (unsigned)sizeof(S);
         }
         S ar[AD][HF][3];
-        // corrector x arms: the (HF-1) columns beyond each end of the rows,
-        // TYP rows; half of these cells per halo-row wave, one per lane.
-        // Idle lanes load a valid cell and write fd column 0, which only
-        // the unstored end lane 0 reads.
-        int xfy = HF, xfx = 0;
-        unsigned xcol = 0;
-        if constexpr (NXH > 0) {
-            const int j = hr * NXH + (lane < NXH ? lane : 0);
-            const int br = j / (2 * (HF - 1)), bc = j - br * (2 * (HF - 1));
-            const int fxa = bc < HF - 1 ? 1 + bc : HF + TX + (bc - (HF - 1));
-            xcol = (colx(x0 - 1 - HF + fxa) + nx * coly(y0 + br)) * (unsigned)sizeof(S);
-            if (lane < NXH) {
-                xfy = HF + br;
-                xfx = fxa;
-            }
-        }
-        S xr[AD][3];
         auto aload = [&](int q, int s) __attribute__((always_inline)) {
             const unsigned so = poff(wrapi(q, nz));
 #pragma unroll
@@ -865,11 +850,6 @@ __device__ __forceinline__ void pipe_item(
                 ar[s][r][0] = bload<S>(rU, acol[r], so);
                 ar[s][r][1] = bload<S>(rV, acol[r], so);
                 ar[s][r][2] = bload<S>(rW, acol[r], so);
-            }
-            if constexpr (NXH > 0) {
-                xr[s][0] = bload<S>(rU, xcol, so);
-                xr[s][1] = bload<S>(rV, xcol, so);
-                xr[s][2] = bload<S>(rW, xcol, so);
             }
         };
 #pragma unroll
@@ -908,16 +888,111 @@ __device__ __forceinline__ void pipe_item(
                     cc.z = ST::ld_u(ar[sa][r][2], P.q);
                     fd[(a - 1) & 1][fy][HF + ix] = cc;
                 }
-                if constexpr (NXH > 0) {
-                    FdCell<R> cc;
-                    cc.x = ST::ld_u(xr[sa][0], P.q);
-                    cc.y = ST::ld_u(xr[sa][1], P.q);
-                    cc.z = ST::ld_u(xr[sa][2], P.q);
-                    fd[(a - 1) & 1][xfy][xfx] = cc;
-                }
             }
             aload(a + 2, k % AD);
             keep(zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]), xv, cen);
+            __syncthreads();
+        };
+        iter(0, 0, false);
+        iter(1, 1, true);
+        for (int ib = 2; ib < nit; ib += U) {
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int t = ib + k;
+                if (t >= nit) break;
+                iter(t, (k + 2) % U, true);
+            }
+        }
+    } else {
+        // ===== halo columns x0-1, x0+64 (lanes < 2*RYP) and the x arms =====
+        const int sl = lane < 2 * RYP ? lane : 2 * RYP - 1;
+        const int side = sl / RYP, r = sl - side * RYP;
+        const int x = side ? x0 + TX : x0 - 1;
+        const int y = y0 - 1 + r;
+        const int ry = r;
+        const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
+        const unsigned col = colx(x) + nx * coly(y);
+        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        const bool writer = lane < 2 * RYP;
+        const unsigned cb_ = col * (unsigned)sizeof(S);
+        S hp[PD], hx[PD], hy[PD], hz[PD];
+        auto hload = [&](int m, int s) __attribute__((always_inline)) {
+            R jump;
+            const unsigned so = poff(phys(m, jump));
+            hp[s] = bload<S>(rP, cb_, so);
+            hx[s] = bload<S>(rU, cb_, so);
+            hy[s] = bload<S>(rV, cb_, so);
+            hz[s] = bload<S>(rW, cb_, so);
+        };
+        // x arms: NXA cells, XPL per lane
+        int afy[XPL], afx[XPL];
+        bool aw[XPL];
+        unsigned acol[XPL];
+#pragma unroll
+        for (int j = 0; j < XPL; j++) {
+            const int cell = lane + 64 * j;
+            aw[j] = cell < NXA;
+            const int cl = aw[j] ? cell : NXA - 1;
+            const int br = cl / (2 * HF), bc = cl - br * (2 * HF);
+            afy[j] = HF + br;
+            afx[j] = bc < HF ? bc : TX + bc;
+            acol[j] = (colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j])) * (unsigned)sizeof(S);
+        }
+        S ar[AD][XPL][3];
+        auto aload = [&](int q, int s) __attribute__((always_inline)) {
+            const unsigned so = poff(wrapi(q, nz));
+#pragma unroll
+            for (int j = 0; j < XPL; j++) {
+                ar[s][j][0] = bload<S>(rU, acol[j], so);
+                ar[s][j][1] = bload<S>(rV, acol[j], so);
+                ar[s][j][2] = bload<S>(rW, acol[j], so);
+            }
+        };
+#pragma unroll
+        for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
+#pragma unroll
+        for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
+        Msg<R> mq[3];
+        mq[0] = msg_of(hp[0], hx[0], hy[0], hz[0], zs - 1, fluid);
+        mq[1] = msg_of(hp[1], hx[1], hy[1], hz[1], zs, fluid);
+        mq[2] = zero_msg<R>();
+        hload(zs - 1 + PD, 0);
+        hload(zs + PD, 1);
+
+        auto iter = [&](int t, int k, bool doB) __attribute__((always_inline)) {
+            const int a = zs + t;
+            {
+                const int s1 = (k + 2) % PD;
+                mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                hload(a + 1 + PD, s1);
+            }
+            if (doB) {
+                const int sa = k % AD;
+#pragma unroll
+                for (int j = 0; j < XPL; j++) {
+                    if (!aw[j]) continue;
+                    FdCell<R> cc;
+                    cc.x = ST::ld_u(ar[sa][j][0], P.q);
+                    cc.y = ST::ld_u(ar[sa][j][1], P.q);
+                    cc.z = ST::ld_u(ar[sa][j][2], P.q);
+                    fd[(a - 1) & 1][afy[j]][afx[j]] = cc;
+                }
+            }
+            aload(a + 2, k % AD);
+            if (writer) {
+                // edge block of this column: the values its neighbour lane
+                // reads (left column -> XL, right column -> XR); two
+                // branches with static indexes (a select between array
+                // elements would become a dynamic index)
+                const XMsg<R> xm = zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]);
+                if (side) {
+#pragma unroll
+                    for (int j = 0; j < 6; j++) edge[a & 1][ry][1][j] = xm.v[XR[j]];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 6; j++) edge[a & 1][ry][0][j] = xm.v[XL[j]];
+                }
+            }
             __syncthreads();
         };
         iter(0, 0, false);
@@ -934,7 +1009,7 @@ __device__ __forceinline__ void pipe_item(
 }
 
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
-__global__ __launch_bounds__((TYP + 2) * 64) void k_step_pipe(
+__global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
@@ -971,7 +1046,7 @@ struct BatchDom {
 // = that + 1 for the next launch (no launch reads and writes the same
 // slot), so hipGraph replays of the two launches keep counting.
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
-__global__ __launch_bounds__((TYP + 2) * 64) void k_step_batch(
+__global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const BatchDom<typename Store<MODE>::R> *__restrict__ doms,
     uint32_t *__restrict__ seeds, const int2 *__restrict__ work, int nwork, int flip)
 {
@@ -996,7 +1071,7 @@ hipError_t launch_batch_mode(const BatchLaunch &b)
     using R = typename Store<MODE>::R;
     constexpr int TYP = pipe_rows(MODE);
     const auto *doms = (const BatchDom<R> *)b.doms;
-    const dim3 grid(b.grid), block((TYP + 2) * 64);
+    const dim3 grid(b.grid), block((TYP + 3) * 64);
 #define FSLBM_BATCH(PER, DD, F2)                                                        \
     k_step_batch<MODE, PER, DD, F2, TYP><<<grid, block, 0, b.stream>>>(doms, b.seeds, b.work, \
                                                                         b.nwork, b.flip)
@@ -1022,12 +1097,11 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
         else return make_consts_f32(a);
     }();
     const int rows = MODE == FSLBM_F64 ? TY : pipe_rows(MODE);
-    const int cols = MODE == FSLBM_F64 ? TX : TXO;
     P.nx = g.nx;
     P.ny = g.ny;
     P.nz = g.nz;
     P.nxy = g.nx * g.ny;
-    P.ntx = (g.nx + cols - 1) / cols;
+    P.ntx = (g.nx + TX - 1) / TX;
     P.nty = (g.ny + rows - 1) / rows;
     P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, MODE);
     if (P.chunk > g.nz) P.chunk = g.nz;
@@ -1066,11 +1140,11 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     else if (P.c2 != R(0))
         k_step_pipe<MODE, PERIODIC, D1, true, pipe_rows(MODE)>
-            <<<grid, (pipe_rows(MODE) + 2) * 64, 0, s>>>(
+            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
                 (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     else  // blend = 0: the order-4 arms have zero weight
         k_step_pipe<MODE, PERIODIC, D1, false, pipe_rows(MODE)>
-            <<<grid, (pipe_rows(MODE) + 2) * 64, 0, s>>>(
+            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
                 (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     return hipGetLastError();
 }
@@ -1096,8 +1170,7 @@ int default_chunk(const Geo &g, int precision)
 {
     // About three work items per CU (256 CUs), chunks of >= 16 planes.
     const int rows = precision == FSLBM_F64 ? TY : pipe_rows(precision);
-    const int cols = precision == FSLBM_F64 ? TX : TXO;
-    const long long tiles = (long long)((g.nx + cols - 1) / cols) * ((g.ny + rows - 1) / rows);
+    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
     long long c = (g.nz * tiles + 767) / 768;
     if (c < 16) c = 16;
     if (c > g.nz) c = g.nz;
