@@ -726,18 +726,30 @@ __device__ __forceinline__ void pipe_item(
         // one iteration; k = t mod 12 (a constant after unrolling)
         auto iter = [&](int t, int k, bool doB, bool doC) __attribute__((always_inline)) {
             const int a = zs + t;
+            // stage C's LDS operands first: their latency overlaps the
+            // message work of stage A
+            R QPp = 0, QPm = 0, QXp = 0, QXm = 0, QZp = 0, QZm = 0;
+            FdCell<R> xp1{}, xm1{}, yp1{}, ym1{};
+            if (doC) {
+                const int cb = (a - 2) & 1;
+                QPp = ymsg[cb][0][ry + 1][ix]; QPm = ymsg[cb][1][ry - 1][ix];
+                QXp = ymsg[cb][2][ry + 1][ix]; QXm = ymsg[cb][3][ry - 1][ix];
+                QZp = ymsg[cb][4][ry + 1][ix]; QZm = ymsg[cb][5][ry - 1][ix];
+                const int cy_ = HF + iy, cx_ = HF + ix;
+                xp1 = fd[cb][cy_][cx_ + D1]; xm1 = fd[cb][cy_][cx_ - D1];
+                yp1 = fd[cb][cy_ + D1][cx_]; ym1 = fd[cb][cy_ - D1][cx_];
+            }
+            __builtin_amdgcn_sched_barrier(0);
             qload(a + HF + LA, (k + QL - 1) % QL);
             {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
                 mq[(k + 2) % 3] = msg_of(qp[s1], qx[s1], qy[s1], qz[s1], a + 1, fluid);
             }
+            __builtin_amdgcn_sched_barrier(0);
             if (doC) {  // stage C: plane c = a-2
                 const int c = a - 2;
                 const int cb = c & 1;
                 const int sc = (k + HF) % QL;
-                const R QPp = ymsg[cb][0][ry + 1][ix], QPm = ymsg[cb][1][ry - 1][ix];
-                const R QXp = ymsg[cb][2][ry + 1][ix], QXm = ymsg[cb][3][ry - 1][ix];
-                const R QZp = ymsg[cb][4][ry + 1][ix], QZm = ymsg[cb][5][ry - 1][ix];
                 constexpr R inv216 = R(1) / R(216);
                 const R ux = ST::ld_u(qx[sc], P.q), uy = ST::ld_u(qy[sc], P.q),
                         uz = ST::ld_u(qz[sc], P.q);
@@ -755,8 +767,6 @@ __device__ __forceinline__ void pipe_item(
                     wy = ps * omy;
                 }
                 const int cy_ = HF + iy, cx_ = HF + ix;
-                const FdCell<R> xp1 = fd[cb][cy_][cx_ + D1], xm1 = fd[cb][cy_][cx_ - D1];
-                const FdCell<R> yp1 = fd[cb][cy_ + D1][cx_], ym1 = fd[cb][cy_ - D1][cx_];
                 const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
                 const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
                 auto U_ = [&](const S *qq, int sl) { return ST::ld_u(qq[sl], P.q); };
