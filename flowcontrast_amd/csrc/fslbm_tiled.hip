@@ -701,20 +701,26 @@ __device__ __forceinline__ void pipe_item(
         const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oU = mkbuf(out + N, fbytes),
                                      oV = mkbuf(out + 2 * N, fbytes), oW = mkbuf(out + 3 * N, fbytes);
         // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
-        S qp[QL], qx[QL], qy[QL], qz[QL];
+        // two fields per entry, (p, ux) and (uy, uz): in the 16-bit
+        // storage modes a pair shares one register
+        typedef S S2 __attribute__((ext_vector_type(2)));
+        S2 qa[QL], qb[QL];
         auto qload = [&](int q, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(q, jump));
-            qp[s] = bload<S>(rP, cb_, so);
-            qx[s] = bload<S>(rU, cb_, so);
-            qy[s] = bload<S>(rV, cb_, so);
-            qz[s] = bload<S>(rW, cb_, so);
+            S2 pa, pb;
+            pa.x = bload<S>(rP, cb_, so);
+            pa.y = bload<S>(rU, cb_, so);
+            pb.x = bload<S>(rV, cb_, so);
+            pb.y = bload<S>(rW, cb_, so);
+            qa[s] = pa;
+            qb[s] = pb;
         };
 #pragma unroll
         for (int j = 2; j < QL - 1; j++) qload(zs - 2 - HF + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(qp[HF + 1], qx[HF + 1], qy[HF + 1], qz[HF + 1], zs - 1, fluid);
-        mq[1] = msg_of(qp[HF + 2], qx[HF + 2], qy[HF + 2], qz[HF + 2], zs, fluid);
+        mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, fluid);
+        mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fluid);
         mq[2] = zero_msg<R>();
         R xv[10], cen[6];
 #pragma unroll
@@ -743,7 +749,7 @@ __device__ __forceinline__ void pipe_item(
             qload(a + HF + LA, (k + QL - 1) % QL);
             {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(qp[s1], qx[s1], qy[s1], qz[s1], a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fluid);
             }
             __builtin_amdgcn_sched_barrier(0);
             if (doC) {  // stage C: plane c = a-2
@@ -751,8 +757,8 @@ __device__ __forceinline__ void pipe_item(
                 const int cb = c & 1;
                 const int sc = (k + HF) % QL;
                 constexpr R inv216 = R(1) / R(216);
-                const R ux = ST::ld_u(qx[sc], P.q), uy = ST::ld_u(qy[sc], P.q),
-                        uz = ST::ld_u(qz[sc], P.q);
+                const R ux = ST::ld_u(qa[sc].y, P.q), uy = ST::ld_u(qb[sc].x, P.q),
+                        uz = ST::ld_u(qb[sc].y, P.q);
                 // unnormalised predictor sums (x216)
                 const R Ps = R(4) * b0c + (QPp + QPm);
                 const R Xs = R(4) * g0c + (QXp + QXm);
@@ -761,7 +767,7 @@ __device__ __forceinline__ void pipe_item(
                 // wall rule terms: own pressure times the wall weights
                 R w0 = 0, wx = 0, wy = 0;
                 if (!PERIODIC) {
-                    const R ps = fluid ? ST::ld_p(qp[sc], c, P.q) : R(0);
+                    const R ps = fluid ? ST::ld_p(qa[sc].x, c, P.q) : R(0);
                     w0 = ps * om0;
                     wx = ps * omx;
                     wy = ps * omy;
@@ -769,11 +775,13 @@ __device__ __forceinline__ void pipe_item(
                 const int cy_ = HF + iy, cx_ = HF + ix;
                 const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
                 const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
-                auto U_ = [&](const S *qq, int sl) { return ST::ld_u(qq[sl], P.q); };
+                auto UX = [&](int sl) { return ST::ld_u(qa[sl].y, P.q); };
+                auto UY = [&](int sl) { return ST::ld_u(qb[sl].x, P.q); };
+                auto UZ = [&](int sl) { return ST::ld_u(qb[sl].y, P.q); };
                 // corrector arm sums at distance d1 (and 2*d1)
-                const R s1x = xp1.x + xm1.x + yp1.x + ym1.x + U_(qx, zp1) + U_(qx, zm1);
-                const R s1y = xp1.y + xm1.y + yp1.y + ym1.y + U_(qy, zp1) + U_(qy, zm1);
-                const R s1z = xp1.z + xm1.z + yp1.z + ym1.z + U_(qz, zp1) + U_(qz, zm1);
+                const R s1x = xp1.x + xm1.x + yp1.x + ym1.x + UX(zp1) + UX(zm1);
+                const R s1y = xp1.y + xm1.y + yp1.y + ym1.y + UY(zp1) + UY(zm1);
+                const R s1z = xp1.z + xm1.z + yp1.z + ym1.z + UZ(zp1) + UZ(zm1);
                 // u' = tb (u*/216 + wall + g (c0 u + c1 S1 + c2 S2)) + (1 - tb) u
                 R nx_ = ux * P.ku + s1x * P.ks1;
                 R ny_ = uy * P.ku + s1y * P.ks1;
@@ -781,9 +789,9 @@ __device__ __forceinline__ void pipe_item(
                 if constexpr (FD2) {  // order-4 arms (blend != 0)
                     const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
                     const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
-                    nx_ += P.ks2 * (xp2.x + xm2.x + yp2.x + ym2.x + U_(qx, zp2) + U_(qx, zm2));
-                    ny_ += P.ks2 * (xp2.y + xm2.y + yp2.y + ym2.y + U_(qy, zp2) + U_(qy, zm2));
-                    nz_ += P.ks2 * (xp2.z + xm2.z + yp2.z + ym2.z + U_(qz, zp2) + U_(qz, zm2));
+                    nx_ += P.ks2 * (xp2.x + xm2.x + yp2.x + ym2.x + UX(zp2) + UX(zm2));
+                    ny_ += P.ks2 * (xp2.y + xm2.y + yp2.y + ym2.y + UY(zp2) + UY(zm2));
+                    nz_ += P.ks2 * (xp2.z + xm2.z + yp2.z + ym2.z + UZ(zp2) + UZ(zm2));
                 }
                 nx_ += P.tb * wx;
                 ny_ += P.tb * wy;
@@ -804,9 +812,9 @@ __device__ __forceinline__ void pipe_item(
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
                 const int sb = (k + 1 + HF) % QL;
                 FdCell<R> cc;
-                cc.x = ST::ld_u(qx[sb], P.q);
-                cc.y = ST::ld_u(qy[sb], P.q);
-                cc.z = ST::ld_u(qz[sb], P.q);
+                cc.x = ST::ld_u(qa[sb].y, P.q);
+                cc.y = ST::ld_u(qb[sb].x, P.q);
+                cc.z = ST::ld_u(qb[sb].y, P.q);
                 fd[(a - 1) & 1][HF + iy][HF + ix] = cc;
             }
             // stage A: plane a
