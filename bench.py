@@ -9,7 +9,10 @@ One step = one FSLBM time step over all 67,108,864 lattice cells.
 
 Printed: one JSON line with MLUPS (all lattice cells, the paper's
 convention), the HBM roofline of the step kernel and the CPU baseline
-(the C oracle with OpenMP on a bounded sample of the same workload).
+(the C oracle with OpenMP on a bounded sample of the same workload), with
+sub-objects for the 256^3 Taylor-Green box (FP32 / FP16 storage against
+FP64), the 64 uneven pipes in one batch, and the serving stream.
+Other workloads: --workload pipe|tgv|batch|serve|stats.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32]
 """
@@ -361,6 +364,49 @@ def bench_stats(steps: int, window: int, every: int, precisions):
     return out
 
 
+def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
+    """Taylor-Green vortex on an n^3 periodic box (U0 = 0.05, Re 1600):
+    step rate and roofline of the unmasked kernel, and the kinetic-energy
+    decay KE(t) and dissipation rate -dKE/dt of each precision against the
+    FP64 run."""
+    from flowcontrast_amd import KERNEL_AUTO, RunParams, Solver, inputs
+    nu = inputs.tgv_viscosity(n, 0.05, 1600.0)
+    prm = RunParams(tblend=1.0, pin=0.0, nu=nu, d1=2, delta=1, blend=0.0)
+    st = inputs.taylor_green_state(n, 0.05, 1.0)
+    cells = n ** 3
+    out = {"workload": f"taylor_green{n}^3, U0 0.05, Re 1600, nu {nu:.4e}, {steps} steps",
+           "cells": cells, "steps": steps}
+    ke = {}
+    for prec in precisions:
+        with Solver((n, n, n), prm, mask=None, precision=prec, kernel=KERNEL_AUTO) as s:
+            s.set_state(*st)
+            series = []
+            for t in range(0, steps + 1, every):
+                if t:
+                    s.step(every)
+                r = s.reduce()  # sums of p, u, v, w, u^2+v^2+w^2, ...
+                series.append(0.5 * r[5] / cells)
+            ke[prec] = np.array(series)
+            if prec != "f64":
+                s.autotune(3)
+                ms = s.time_steps(timed_steps) / timed_steps
+                B = algorithmic_bytes_per_cell(prec, False)
+                out[f"{prec}_mlups"] = round(cells / ms / 1e3, 1)
+                out[f"{prec}_roofline_frac"] = round(B * cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    ref = ke["f64"]
+    eps_ref = -np.diff(ref) / every
+    out["ke0"] = float(ref[0])
+    out["ke_final_over_ke0_f64"] = float(ref[-1] / ref[0])
+    for prec in precisions:
+        if prec == "f64":
+            continue
+        eps = -np.diff(ke[prec]) / every
+        out[f"{prec}_ke_max_rel_err"] = float(f"{float(np.max(np.abs(ke[prec] - ref) / ref)):.3e}")
+        out[f"{prec}_dissipation_max_rel_err"] = float(
+            f"{float(np.max(np.abs(eps - eps_ref)) / np.max(np.abs(eps_ref))):.3e}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +423,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "pipe", "batch", "serve", "stats"],
+                    choices=["all", "pipe", "batch", "serve", "stats", "tgv"],
                     help="all: the pipe step line with the batched and serving "
                          "results attached")
     ap.add_argument("--batch-count", type=int, default=64)
@@ -411,6 +457,9 @@ def main() -> int:
     if args.workload == "batch":
         print(json.dumps(bench_batch(args.batch_count, args.batch_steps,
                                      not args.no_sequential)), flush=True)
+        return 0
+    if args.workload == "tgv":
+        print(json.dumps(bench_tgv(256, 5000, 250, 100, ["f64", "f32", "f16s"])), flush=True)
         return 0
     if args.workload == "stats":
         print(json.dumps(bench_stats(args.stats_steps, args.stats_window, args.stats_every,
@@ -498,7 +547,9 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(mask, st, pin, nu, args.cpu_steps,
                                            args.precision)
     if args.workload == "all":
-        for key, fn in (("batched", lambda: bench_batch(args.batch_count, args.batch_steps,
+        for key, fn in (("taylor_green", lambda: bench_tgv(256, 5000, 250, 100,
+                                                          ["f64", "f32", "f16s"])),
+                        ("batched", lambda: bench_batch(args.batch_count, args.batch_steps,
                                                        not args.no_sequential)),
                         ("serving", lambda: bench_serve(args.serve_requests, args.serve_rate,
                                                         args.serve_quantum, args.serve_budget,

@@ -188,3 +188,30 @@ def test_plain_and_tiled_agree_f32():
         pytest.skip("only one kernel variant available")
     # two FP32 evaluation orders: roundoff only, within the FP32 tolerance
     assert max(errors_l2(outs[0], outs[1])) <= 1e-5
+
+
+@pytest.mark.parametrize("precision, tol", [("f32", 5e-5), ("f16s", 2e-3), ("bf16s", 1e-2)])
+def test_taylor_green_kinetic_energy_decay(precision, tol):
+    """64^3 Taylor-Green vortex (U0 0.05, Re 400), 1000 steps: the kinetic
+    energy history of each storage mode follows the FP64 run.  FP32 runs
+    the reference's float literals (1/6 as 0.16666666f, weights to 9-11
+    digits), whose effective viscosity and weight sum differ from the exact
+    rationals of FP64 by ~1e-8, on top of 1000 steps of float rounding."""
+    n, steps, every = 64, 1000, 100
+    nu = inputs.tgv_viscosity(n, 0.05, 400.0)
+    prm = RunParams(tblend=1.0, pin=0.0, nu=nu, d1=2, delta=1, blend=0.0)
+    st = inputs.taylor_green_state(n, 0.05, 1.0)
+
+    def history(prec):
+        with Solver((n, n, n), prm, mask=None, precision=prec) as s:
+            s.set_state(*st)
+            ke = []
+            for t in range(0, steps + 1, every):
+                if t:
+                    s.step(every)
+                ke.append(s.reduce()[5])
+            return np.array(ke)
+
+    ref, got = history("f64"), history(precision)
+    assert ref[-1] < 0.9 * ref[0]  # the decay to be followed
+    assert np.max(np.abs(got - ref) / ref) < tol
