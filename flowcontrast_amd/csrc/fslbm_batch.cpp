@@ -34,6 +34,8 @@ struct Domain {
     size_t bytes = 0;
     int parity = 0;  // buffer holding the current state
     long steps = 0;
+    bool held = false;  // resident but no longer stepped (result pending)
+    int held_cur = 0;   // its current buffer
 };
 
 constexpr int NCLASS = 4;  // (periodic, order-4 arms)
@@ -120,7 +122,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     std::vector<int2> work[NCLASS];
     for (size_t i = 0; i < nd; i++) {
         Domain &d = b->doms[i];
-        if (!d.active) continue;
+        if (!d.active || d.held) continue;
         // the current parity of each domain is folded into its descriptor
         d.parity ^= b->flip;
         const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
@@ -236,7 +238,7 @@ int step_impl(fslbm_batch *b, int nsteps)
         b->flip ^= 1;
     }
     for (auto &d : b->doms)
-        if (d.active) d.steps += nsteps;
+        if (d.active && !d.held) d.steps += nsteps;
     return FSLBM_OK;
 }
 
@@ -304,8 +306,9 @@ int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, 
 
 // Upload a prepared domain and admit it.  Returns the slot, FSLBM_EBUSY
 // when the budget cannot hold it, or another error.
-int batch_attach(fslbm_batch *b, PreparedDomain &pd)
+int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
 {
+    if (!s && b) s = b->stream;
     if (!b) return FSLBM_EINVAL;
     if (pd.precision != b->precision)
         return bfail(b, FSLBM_EINVAL, "domain was prepared for another precision mode");
@@ -333,19 +336,21 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd)
     if (ok && !g.periodic) {
         ok = hipMalloc((void **)&d.mask2d, nxy) == hipSuccess &&
              hipMalloc(&d.omega, pd.omega.size() * sizeof(float)) == hipSuccess &&
-             hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipMemcpyAsync(d.omega, pd.omega.data(), pd.omega.size() * sizeof(float), hipMemcpyHostToDevice, b->stream) == hipSuccess;
+             hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(d.omega, pd.omega.data(), pd.omega.size() * sizeof(float), hipMemcpyHostToDevice, s) == hipSuccess;
     }
     if (ok)
-        ok = hipMemcpyAsync(d.buf[0], pd.host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipMemcpyAsync(d.buf[1], pd.host.data(), sbytes, hipMemcpyHostToDevice, b->stream) == hipSuccess &&
-             hipStreamSynchronize(b->stream) == hipSuccess;
+        ok = hipMemcpyAsync(d.buf[0], pd.host.data(), sbytes, hipMemcpyHostToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(d.buf[1], pd.host.data(), sbytes, hipMemcpyHostToDevice, s) == hipSuccess &&
+             (!sync || hipStreamSynchronize(s) == hipSuccess);
     if (!ok) {
         cleanup();
         return bfail(b, FSLBM_ENOMEM, "device allocation for the domain failed");
     }
-    pd.host = std::vector<unsigned char>();
-    pd.omega = std::vector<float>();
+    if (sync) {
+        pd.host = std::vector<unsigned char>();
+        pd.omega = std::vector<float>();
+    }  // else the caller keeps them until it has synchronised s
     // reuse a free slot
     int slot = -1;
     for (size_t i = 0; i < b->doms.size(); i++)
@@ -370,7 +375,7 @@ int batch_add_plane(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask
     PreparedDomain pd;
     const int rc = batch_prepare(b, prm, mask, nx, ny, nz, src, pd);
     if (rc) return rc;
-    return batch_attach(b, pd);
+    return batch_attach(b, pd, nullptr, true);
 }
 
 }  // namespace fslbm
@@ -493,7 +498,7 @@ int fslbm_batch_get_state(fslbm_batch_t b, int slot, double *p, double *ux, doub
     if (!p || !ux || !uy || !uz) return bfail(b, FSLBM_EINVAL, "NULL state array");
     const Domain &d = b->doms[slot];
     // current buffer: parity ^ flip (a new domain holds its state in both)
-    const int cur = d.parity ^ b->flip;
+    const int cur = d.held ? d.held_cur : d.parity ^ b->flip;
     const size_t sbytes = 4 * (size_t)d.g.N * storage_bytes(b->precision);
     std::vector<unsigned char> host(sbytes);
     BCHK(b, hipMemcpyAsync(host.data(), d.buf[cur], sbytes, hipMemcpyDeviceToHost, b->stream));
@@ -601,15 +606,49 @@ int fslbm_batch_sync(fslbm_batch_t b)
 
 namespace fslbm {
 
+// Stop stepping a domain but keep it resident (its result is copied out
+// later, while the others step).
+int batch_hold(fslbm_batch *b, int slot)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
+        return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
+    Domain &d = b->doms[slot];
+    if (d.held) return FSLBM_OK;
+    d.held_cur = d.parity ^ b->flip;
+    d.held = true;
+    b->dirty = true;
+    return FSLBM_OK;
+}
+
+// Enqueue the copy of a held domain's state to out.raw on stream s (the
+// caller synchronises s before reading out.raw or removing the domain).
+int batch_copy_out(fslbm_batch *b, int slot, HostResult &out, hipStream_t s)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active || !b->doms[slot].held)
+        return bfail(b, FSLBM_EINVAL, "no held domain in slot %d", slot);
+    const Domain &d = b->doms[slot];
+    const size_t sbytes = 4 * (size_t)d.g.N * storage_bytes(b->precision);
+    out.g = d.g;
+    out.precision = b->precision;
+    out.q = d.q;
+    out.mask2d = d.hmask2d;
+    out.steps = d.steps;
+    out.raw.resize(sbytes);
+    BCHK(b, hipMemcpyAsync(out.raw.data(), d.buf[d.held_cur], sbytes, hipMemcpyDeviceToHost, s));
+    return FSLBM_OK;
+}
+
 // Copy a domain's current stored state to the host and remove it from the
-// batch (the serving loop's hand-back of a finished solve).
+// batch.
 int batch_take(fslbm_batch *b, int slot, HostResult &out)
 {
     if (!b) return FSLBM_EINVAL;
     if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
         return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
     const Domain &d = b->doms[slot];
-    const int cur = d.parity ^ b->flip;
+    const int cur = d.held ? d.held_cur : d.parity ^ b->flip;
     const size_t sbytes = 4 * (size_t)d.g.N * storage_bytes(b->precision);
     out.g = d.g;
     out.precision = b->precision;

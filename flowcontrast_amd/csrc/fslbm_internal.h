@@ -98,7 +98,13 @@ struct PreparedDomain;  // fslbm_codec.h
 // and admit it; batch_add_plane does both.
 int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
                   int ny, int nz, const double *const src[4], PreparedDomain &pd);
-int batch_attach(fslbm_batch *b, PreparedDomain &pd);
+// attach uploads on stream s (nullptr: the batch's) and waits for it if
+// sync; otherwise the caller keeps pd's host buffers until it has.
+int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync);
+// Stop stepping a domain, keeping it resident; then copy its state out on
+// a stream of the caller's (asynchronous).
+int batch_hold(fslbm_batch *b, int slot);
+int batch_copy_out(fslbm_batch *b, int slot, HostResult &out, hipStream_t s);
 // Admit a domain whose mask (NULL: periodic box) is one z plane of nx*ny
 // codes; src holds the four initial fields.  Returns the slot or an error.
 int batch_add_plane(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,

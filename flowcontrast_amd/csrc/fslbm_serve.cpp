@@ -1,19 +1,21 @@
 // Serving loop: solve requests (pipes, periodic boxes) of different sizes,
 // run parameters, precision modes and step budgets are queued as they
 // arrive and run as batch domains.  One pump:
-//   1. launches up to `quantum` steps of every resident request (one
+//   1. launches up to `quantum` steps of every stepping request (one
 //      batched launch sequence per precision mode, each mode on its own
 //      stream, so that they overlap);
-//   2. while those run, validates and encodes queued requests on the host,
-//      in arrival order, while the HBM budget holds them (memory of
-//      residents that finish in this pump counts as free);
-//   3. waits for the steps;
-//   4. hands finished requests back (state copied to the host, device
-//      memory freed);
-//   5. uploads the encoded requests, which step from the next pump on.
+//   2. while those run, copies out the results of the requests that
+//      finished in the previous pump (they stay resident, no longer
+//      stepped), encodes queued requests on the host in arrival order while
+//      the HBM budget holds them, and uploads them, all on a copy stream;
+//   3. waits for both;
+//   4. frees the copied-out requests and hands them back, and lets the
+//      uploaded ones step from the next pump on;
+//   5. holds the requests whose step budget is now spent.
 // The pump's step count for a mode is the smallest remaining budget of
-// its residents, capped by the quantum, so no request runs past its
-// budget.
+// its stepping requests, capped by the quantum, so no request runs past
+// its budget.  Device memory of a request counts against the budget from
+// its upload to its hand-back.
 
 #include <hip/hip_runtime.h>
 
@@ -46,7 +48,7 @@ int mode_index(int precision)
     }
 }
 
-enum class RState { Queued, Running, Done, Failed };
+enum class RState { Queued, Ready, Running, Held, Done, Failed };
 
 struct Request {
     fslbm_request r{};
@@ -73,8 +75,10 @@ struct fslbm_server {
     size_t budget = 0, used = 0;
     fslbm_batch_t batch[NMODES] = {nullptr, nullptr, nullptr};
     std::deque<long long> queue;        // arrival order
-    std::vector<long long> ready;       // encoded, waiting for upload
-    std::vector<long long> running;     // admitted, in admission order
+    std::vector<long long> ready;       // encoded, uploading in this pump
+    std::vector<long long> running;     // stepping, in admission order
+    std::vector<long long> held;        // finished, result copied out next pump
+    hipStream_t copy = nullptr;         // result copies and uploads
     std::map<long long, Request> reqs;  // every request not yet dropped
     std::deque<long long> unreported;   // finished, not yet popped
     long long next_id = 0;
@@ -126,17 +130,16 @@ void finish(fslbm_server *s, long long id, Request &q, RState st, int status, do
 }
 
 // Encode queued requests in arrival order while the budget holds them
-// (host work, run while the pump's steps execute).  `freeing` is the
-// device memory of residents that finish in this pump: it is released
-// before any prepared request is uploaded.  Returns the number of
-// requests that failed validation, or an error.
-int prepare_queued(fslbm_server *s, size_t freeing)
+// (host work, run while the pump's steps execute); memory of held results
+// still counts, as they are freed only after this pump's copies.  Returns
+// the number of requests that failed validation, or an error.
+int prepare_queued(fslbm_server *s)
 {
     int failed = 0;
     while (!s->queue.empty()) {
         const long long id = s->queue.front();
         Request &q = s->reqs.at(id);
-        if (s->used - freeing + q.bytes > s->budget) break;  // FIFO: no overtaking
+        if (s->used + q.bytes > s->budget) break;  // FIFO: no overtaking
         int rc = ensure_batch(s, q.mode);
         if (rc) return rc;
         const double *src[4] = {q.st[0].data(), q.st[1].data(), q.st[2].data(), q.st[3].data()};
@@ -152,31 +155,9 @@ int prepare_queued(fslbm_server *s, size_t freeing)
             continue;
         }
         s->used += q.bytes;  // reserved until the request is handed back
+        q.state = RState::Ready;
         s->ready.push_back(id);
     }
-    return failed;
-}
-
-// Upload the prepared requests and make them resident.
-int attach_ready(fslbm_server *s)
-{
-    int failed = 0;
-    for (long long id : s->ready) {
-        Request &q = s->reqs.at(id);
-        const int rc = batch_attach(s->batch[q.mode], q.prep);
-        q.prep = PreparedDomain();
-        if (rc < 0) {
-            s->used -= q.bytes;
-            finish(s, id, q, RState::Failed, rc, s->now());
-            failed++;
-            continue;
-        }
-        q.slot = rc;
-        q.state = RState::Running;
-        q.t_admit = s->now();
-        s->running.push_back(id);
-    }
-    s->ready.clear();
     return failed;
 }
 
@@ -196,6 +177,10 @@ int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return sfail(nullptr, FSLBM_ENODEV, "no HIP device visible");
     fslbm_server *s = new fslbm_server();
+    if (hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return sfail(nullptr, FSLBM_EHIP, "stream creation failed");
+    }
     s->d1 = d1;
     s->quantum = quantum;
     s->budget = hbm_budget;
@@ -207,8 +192,10 @@ int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *
 void fslbm_server_destroy(fslbm_server_t s)
 {
     if (!s) return;
+    if (s->copy) (void)hipStreamSynchronize(s->copy);
     for (auto &b : s->batch)
         if (b) fslbm_batch_destroy(b);
+    if (s->copy) (void)hipStreamDestroy(s->copy);
     delete s;
 }
 
@@ -262,38 +249,69 @@ long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req, const 
 int fslbm_server_pump(fslbm_server_t s)
 {
     if (!s) return FSLBM_EINVAL;
-    // 1. launch the steps of the residents (asynchronous, one stream per mode)
+    // 1. launch the steps of the stepping requests (one stream per mode)
     long long qs[NMODES] = {0, 0, 0};
     for (long long id : s->running) {
         const Request &q = s->reqs.at(id);
         long long &m = qs[q.mode];
         m = m == 0 ? q.left : std::min(m, q.left);
     }
-    size_t freeing = 0;
-    for (int k = 0; k < NMODES; k++) qs[k] = std::min<long long>(qs[k], s->quantum);
-    for (long long id : s->running) {
-        const Request &q = s->reqs.at(id);
-        if (q.left == qs[q.mode]) freeing += q.bytes;
-    }
     int rc;
-    for (int k = 0; k < NMODES; k++)
+    for (int k = 0; k < NMODES; k++) {
+        qs[k] = std::min<long long>(qs[k], s->quantum);
         if (qs[k] > 0) {
             rc = fslbm_batch_step(s->batch[k], (int)qs[k]);
             if (rc) return sfail(s, rc, "batch step: %s", fslbm_batch_last_error(s->batch[k]));
         }
-    // 2. meanwhile encode the queued requests the budget admits
-    rc = prepare_queued(s, freeing);
+    }
+    // 2. meanwhile: copy out last pump's results, encode and upload queued
+    // requests (copy stream)
+    for (long long id : s->held) {
+        Request &q = s->reqs.at(id);
+        rc = batch_copy_out(s->batch[q.mode], q.slot, q.res, s->copy);
+        if (rc) return sfail(s, rc, "result copy: %s", fslbm_batch_last_error(s->batch[q.mode]));
+    }
+    rc = prepare_queued(s);
     if (rc < 0) return rc;
     int finished = rc;
-    // 3. wait for the steps
+    std::vector<long long> uploaded;
+    for (long long id : s->ready) {
+        Request &q = s->reqs.at(id);
+        rc = batch_attach(s->batch[q.mode], q.prep, s->copy, false);
+        if (rc < 0) {
+            q.prep = PreparedDomain();
+            s->used -= q.bytes;
+            finish(s, id, q, RState::Failed, rc, s->now());
+            finished++;
+            continue;
+        }
+        q.slot = rc;
+        uploaded.push_back(id);
+    }
+    s->ready.clear();
+    // 3. wait for the steps and the copies
     for (int k = 0; k < NMODES; k++)
         if (qs[k] > 0) {
             rc = fslbm_batch_sync(s->batch[k]);
             if (rc) return sfail(s, rc, "batch sync: %s", fslbm_batch_last_error(s->batch[k]));
         }
-    // 4. hand back what finished (admission order), freeing its memory
+    if (hipStreamSynchronize(s->copy) != hipSuccess)
+        return sfail(s, FSLBM_EHIP, "copy stream failed");
+    const double t = s->now();
+    // 4. hand back the copied-out results, freeing their memory
+    for (long long id : s->held) {
+        Request &q = s->reqs.at(id);
+        rc = fslbm_batch_remove(s->batch[q.mode], q.slot);
+        if (rc) return sfail(s, rc, "remove: %s", fslbm_batch_last_error(s->batch[q.mode]));
+        s->used -= q.bytes;
+        q.slot = -1;
+        finish(s, id, q, RState::Done, FSLBM_OK, t);
+        finished++;
+    }
+    s->held.clear();
+    // 5. hold the requests whose budget is now spent (no further steps)
     std::vector<long long> still;
-    still.reserve(s->running.size());
+    still.reserve(s->running.size() + uploaded.size());
     for (long long id : s->running) {
         Request &q = s->reqs.at(id);
         q.left -= qs[q.mode];
@@ -301,18 +319,21 @@ int fslbm_server_pump(fslbm_server_t s)
             still.push_back(id);
             continue;
         }
-        rc = batch_take(s->batch[q.mode], q.slot, q.res);
-        if (rc) return sfail(s, rc, "result copy: %s", fslbm_batch_last_error(s->batch[q.mode]));
-        s->used -= q.bytes;
-        q.slot = -1;
-        finish(s, id, q, RState::Done, FSLBM_OK, s->now());
-        finished++;
+        rc = batch_hold(s->batch[q.mode], q.slot);
+        if (rc) return sfail(s, rc, "hold: %s", fslbm_batch_last_error(s->batch[q.mode]));
+        q.state = RState::Held;
+        s->held.push_back(id);
+    }
+    // the uploaded requests step from the next pump on
+    for (long long id : uploaded) {
+        Request &q = s->reqs.at(id);
+        q.prep = PreparedDomain();  // host copy no longer needed
+        q.state = RState::Running;
+        q.t_admit = t;
+        still.push_back(id);
     }
     s->running.swap(still);
-    // 5. upload the prepared requests: they step from the next pump on
-    rc = attach_ready(s);
-    if (rc < 0) return rc;
-    return finished + rc;
+    return finished;
 }
 
 int fslbm_server_next_done(fslbm_server_t s, fslbm_completion *out)
@@ -369,7 +390,7 @@ int fslbm_server_query(fslbm_server_t s, int *queued, int *running, int *finishe
     for (const auto &kv : s->reqs)
         if (kv.second.state == RState::Done || kv.second.state == RState::Failed) fin++;
     if (queued) *queued = (int)(s->queue.size() + s->ready.size());
-    if (running) *running = (int)s->running.size();
+    if (running) *running = (int)(s->running.size() + s->held.size());
     if (finished) *finished = fin;
     if (used) *used = s->used;
     if (budget) *budget = s->budget;
