@@ -3,7 +3,8 @@
 // arrive and run as batch domains.  One pump:
 //   1. launches up to `quantum` steps of every stepping request (one
 //      batched launch sequence per precision mode, each mode on its own
-//      stream, so that they overlap);
+//      stream: the modes' launches overlap, which measured faster than
+//      running them one after another);
 //   2. while those run, copies out the results of the requests that
 //      finished in the previous pump (they stay resident, no longer
 //      stepped), encodes queued requests on the host in arrival order while
@@ -79,6 +80,7 @@ struct fslbm_server {
     std::vector<long long> running;     // stepping, in admission order
     std::vector<long long> held;        // finished, result copied out next pump
     hipStream_t copy = nullptr;         // result copies and uploads
+    double last_steps_s = 0.0;          // wall time of the last pump's steps
     std::map<long long, Request> reqs;  // every request not yet dropped
     std::deque<long long> unreported;   // finished, not yet popped
     long long next_id = 0;
@@ -131,12 +133,19 @@ void finish(fslbm_server *s, long long id, Request &q, RState st, int status, do
 
 // Encode queued requests in arrival order while the budget holds them
 // (host work, run while the pump's steps execute); memory of held results
-// still counts, as they are freed only after this pump's copies.  Returns
-// the number of requests that failed validation, or an error.
+// still counts, as they are freed only after this pump's copies.  The host
+// time spent is bounded by the last pump's step time (at least one
+// request), so the encoding stays hidden behind the steps and the GPU is
+// not left waiting on a long admission.  Returns the number of requests
+// that failed validation, or an error.
 int prepare_queued(fslbm_server *s)
 {
     int failed = 0;
+    const double t0 = s->now();
+    const double budget_s = std::max(s->last_steps_s, 0.005);
+    int prepared = 0;
     while (!s->queue.empty()) {
+        if (prepared > 0 && s->now() - t0 > budget_s) break;
         const long long id = s->queue.front();
         Request &q = s->reqs.at(id);
         if (s->used + q.bytes > s->budget) break;  // FIFO: no overtaking
@@ -157,6 +166,7 @@ int prepare_queued(fslbm_server *s)
         s->used += q.bytes;  // reserved until the request is handed back
         q.state = RState::Ready;
         s->ready.push_back(id);
+        prepared++;
     }
     return failed;
 }
@@ -257,6 +267,7 @@ int fslbm_server_pump(fslbm_server_t s)
         m = m == 0 ? q.left : std::min(m, q.left);
     }
     int rc;
+    const double t_launch = s->now();
     for (int k = 0; k < NMODES; k++) {
         qs[k] = std::min<long long>(qs[k], s->quantum);
         if (qs[k] > 0) {
@@ -298,6 +309,9 @@ int fslbm_server_pump(fslbm_server_t s)
     if (hipStreamSynchronize(s->copy) != hipSuccess)
         return sfail(s, FSLBM_EHIP, "copy stream failed");
     const double t = s->now();
+    bool stepped = false;
+    for (int k = 0; k < NMODES; k++) stepped |= qs[k] > 0;
+    if (stepped) s->last_steps_s = t - t_launch;
     // 4. hand back the copied-out results, freeing their memory
     for (long long id : s->held) {
         Request &q = s->reqs.at(id);
