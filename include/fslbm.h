@@ -285,14 +285,15 @@ int fslbm_batch_sync(fslbm_batch_t b);
 /* ------------------------------------------------------------------ */
 /* Serving loop (no counterpart in the reference): solve requests of    */
 /* different sizes, run parameters and step budgets arrive at any time. */
-/* Each pump advances every resident request by up to `quantum` steps   */
+/* Each pump advances every stepping request by up to `quantum` steps   */
 /* in one batched launch sequence per precision mode (the modes overlap */
-/* on separate streams); meanwhile it encodes queued requests on the    */
-/* host in arrival order while the HBM budget holds them; then it hands */
-/* finished requests back (state copied to the host, device memory      */
-/* freed) and uploads the encoded ones, which join the residents at the */
-/* next pump (continuous batching).  A request's result is bitwise the  */
-/* same as a solo run of the tiled single-domain path in its precision. */
+/* on separate streams).  Meanwhile it copies out the results of the    */
+/* requests whose budget ran out in the previous pump, and encodes and  */
+/* uploads queued requests in arrival order while the HBM budget holds  */
+/* them; the uploaded requests join the stepping ones at the next pump  */
+/* (continuous batching).  A request holds device memory from upload to */
+/* hand-back.  Its result is bitwise the same as a solo run of the      */
+/* tiled single-domain path in its precision mode.                     */
 /* ------------------------------------------------------------------ */
 typedef struct fslbm_server *fslbm_server_t;
 
