@@ -190,7 +190,7 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
     hand-back.  Then the same requests submitted at once (capacity) and
     one after another through the single-domain solver; the smallest
     requests are checked against solo runs (bitwise) and FP64 runs."""
-    from flowcontrast_amd import KERNEL_AUTO, Server, Solver
+    from flowcontrast_amd import KERNEL_AUTO, KERNEL_PLAIN, Server, Solver
     from flowcontrast_amd import workloads as W
     cases = W.serving_requests(count, 5, rate)
     built = [c.build() for c in cases]
@@ -268,21 +268,32 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
         # the same requests one after another through the single-domain
         # drop-in, each timed like a served one: state upload, steps and
         # result download (the served capacity run includes all three)
-        seq = seq_steps = 0.0
+        # The plain one-thread-per-cell kernel is timed on up to 300 steps per
+        # request and its step time scaled to the request's budget.
+        seq = seq_steps = seq_plain = 0.0
         for c, (prm, mask, st) in zip(cases, built):
-            t0 = time.perf_counter()
-            with Solver(c.shape, prm, mask=mask, precision=c.precision, kernel=KERNEL_AUTO) as s:
-                s.set_state(*st)
-                s.sync()
-                t1 = time.perf_counter()
-                s.step(c.steps)
-                s.sync()
-                seq_steps += time.perf_counter() - t1
-                s.get_state()
-            seq += time.perf_counter() - t0
+            for kern in (KERNEL_AUTO, KERNEL_PLAIN):
+                n = c.steps if kern == KERNEL_AUTO else min(c.steps, 300)
+                t0 = time.perf_counter()
+                with Solver(c.shape, prm, mask=mask, precision=c.precision, kernel=kern) as s:
+                    s.set_state(*st)
+                    s.sync()
+                    t1 = time.perf_counter()
+                    s.step(n)
+                    s.sync()
+                    dt_steps = time.perf_counter() - t1
+                    s.get_state()
+                dt = time.perf_counter() - t0
+                if kern == KERNEL_AUTO:
+                    seq += dt
+                    seq_steps += dt_steps
+                else:
+                    seq_plain += dt - dt_steps + dt_steps * c.steps / n
         out["sequential_solves_per_s"] = round(count / seq, 4)
         out["sequential_steps_only_solves_per_s"] = round(count / seq_steps, 4)
         out["capacity_vs_sequential"] = round(seq / end2, 3)
+        out["sequential_plain_solves_per_s"] = round(count / seq_plain, 4)
+        out["capacity_vs_sequential_plain"] = round(seq_plain / end2, 3)
     return out
 
 
