@@ -13,6 +13,13 @@
 //        p ux uy uz IO .txt (YZ plane x=lx/2), ux1 uy1 uz1 p1 .txt (XY
 //                   plane z=lz/2) every 500 steps (:772-821)
 //        p3Dii ... _avii.txt, qpii.txt backup every 50000 steps (:825-904)
+//        fslbm_ckpt.bin  exact checkpoint written with each backup: the
+//                   state and the statistics sums as doubles, the step
+//                   number and the pending gauge.  A run started without
+//                   --fresh resumes from it when present (the reference's
+//                   float32 volumes and %e averages are the fallback), and
+//                   continues the step numbering, so an interrupted run
+//                   repeats a continuous one bitwise (FP64 and FP32 modes).
 // The geometry sizes, hard-coded in the reference (FSLBM.cu:512-514), are
 // options here.  param.txt is re-read every 500 steps (FSLBM.cu:669-677).
 //
@@ -129,6 +136,85 @@ void copy_file(const char *src, const char *dst)
     std::fclose(b);
 }
 
+constexpr char CKPT_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'C', 'K', '1'};
+
+struct CkptHeader {
+    char magic[8];
+    int32_t nx, ny, nz, precision;
+    int64_t step;
+    double gauge;
+    int32_t stats_n, cnt, cnt2, pad;
+};
+
+void write_checkpoint(fslbm_t ctx, int nx, int ny, int nz, int precision, long step,
+                      double gauge)
+{
+    const size_t N = (size_t)nx * ny * nz;
+    std::vector<double> st[4];
+    for (auto &v : st) v.resize(N);
+    check(fslbm_get_state(ctx, st[0].data(), st[1].data(), st[2].data(), st[3].data()), ctx,
+          "checkpoint state");
+    CkptHeader h{};
+    std::memcpy(h.magic, CKPT_MAGIC, 8);
+    h.nx = nx;
+    h.ny = ny;
+    h.nz = nz;
+    h.precision = precision;
+    h.step = step;
+    h.gauge = gauge;
+    h.stats_n = fslbm_stats_points(ctx);
+    std::vector<double> acc(11 * (size_t)h.stats_n);
+    int cnt = 0, cnt2 = 0;
+    check(fslbm_stats_read(ctx, acc.data(), &cnt, &cnt2), ctx, "checkpoint stats");
+    h.cnt = cnt;
+    h.cnt2 = cnt2;
+    FILE *f = std::fopen("fslbm_ckpt.bin.tmp", "wb");
+    if (!f) die("cannot write fslbm_ckpt.bin");
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+    for (auto &v : st) ok = ok && std::fwrite(v.data(), sizeof(double), N, f) == N;
+    ok = ok && std::fwrite(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok || std::rename("fslbm_ckpt.bin.tmp", "fslbm_ckpt.bin") != 0)
+        die("writing fslbm_ckpt.bin failed");
+}
+
+// Returns the step number of the checkpoint (state, statistics and gauge
+// loaded into ctx), or -1 when there is none.
+long read_checkpoint(fslbm_t ctx, int nx, int ny, int nz, int precision)
+{
+    FILE *f = std::fopen("fslbm_ckpt.bin", "rb");
+    if (!f) return -1;
+    CkptHeader h{};
+    if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, CKPT_MAGIC, 8) != 0) {
+        std::fclose(f);
+        die("fslbm_ckpt.bin: not a checkpoint");
+    }
+    if (h.nx != nx || h.ny != ny || h.nz != nz) {
+        std::fclose(f);
+        die("fslbm_ckpt.bin: written for another geometry");
+    }
+    if (h.precision != precision)
+        std::fprintf(stderr, "fslbm_run: checkpoint written in precision mode %d, resuming in %d\n",
+                     h.precision, precision);
+    const size_t N = (size_t)nx * ny * nz;
+    std::vector<double> st[4];
+    bool ok = true;
+    for (auto &v : st) {
+        v.resize(N);
+        ok = ok && std::fread(v.data(), sizeof(double), N, f) == N;
+    }
+    if (h.stats_n != fslbm_stats_points(ctx)) ok = false;
+    std::vector<double> acc(11 * (size_t)(h.stats_n > 0 ? h.stats_n : 0));
+    ok = ok && std::fread(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    std::fclose(f);
+    if (!ok) die("fslbm_ckpt.bin: truncated");
+    check(fslbm_set_state(ctx, st[0].data(), st[1].data(), st[2].data(), st[3].data()), ctx,
+          "checkpoint state");
+    check(fslbm_stats_load(ctx, acc.data(), h.cnt, h.cnt2), ctx, "checkpoint stats");
+    if (h.gauge != 0.0) check(fslbm_set_gauge(ctx, h.gauge), ctx, "checkpoint gauge");
+    return (long)h.step;
+}
+
 const char *const AVG_NAMES[11] = {"_p", "_ux", "_uy", "_uz", "_p2", "_ux2",
                                    "_uy2", "_uz2", "_uxy2", "_uxz2", "_uyz2"};
 
@@ -233,8 +319,11 @@ int main(int argc, char **argv)
     std::vector<double> st[4];
     for (auto &v : st) v.assign(N, 0.0);
     const char *vol[4] = {"p3Dii", "ux3Dii", "uy3Dii", "uz3Dii"};
-    const bool restart = !o.fresh && file_exists(vol[0]);
-    if (restart) {
+    const bool exact = !o.fresh && file_exists("fslbm_ckpt.bin");
+    const bool restart = !exact && !o.fresh && file_exists(vol[0]);
+    if (exact) {
+        copy_file("qpii.txt", "qp.txt");
+    } else if (restart) {
         const size_t lxy = (size_t)o.lx * o.ly;
         std::vector<float> plane(lxy);
         for (int f = 0; f < 4; f++) {
@@ -259,17 +348,21 @@ int main(int argc, char **argv)
     for (auto &v : st) std::vector<double>().swap(v);
     check(fslbm_stats_config(ctx, o.lx / 2, o.d2), ctx, "stats config");
     if (restart) load_averages(ctx);
+    long c0 = 0;  // steps done before this run (exact checkpoint)
+    if (exact) c0 = read_checkpoint(ctx, nx, ny, nz, o.precision);
     { FILE *q = std::fopen("qp.txt", "a"); if (q) std::fclose(q); }
 
     const int n_yz = ny * o.lz;
     std::vector<double> yz(n_yz), xy(nxy);
-    for (long c = 1; c <= o.steps; c++) {
-        check(fslbm_step(ctx, 1), ctx, "step");
-        if (c % o.monitor != 0) continue;
+    double pending_gauge = 0.0;  // p2 set by the last monitoring step
+    // FSLBM.cu:630-764: monitoring, and every `stats` steps the statistics
+    // and plane files
+    auto monitor_step = [&](long c) {
         // FSLBM.cu:630-666
         double m[4];
         check(fslbm_monitor(ctx, 1, m), ctx, "monitor");
         const double q1 = m[0], q2 = m[1], p1 = m[2], p2 = m[3];
+        pending_gauge = p2;
         const double Re = 2.0 * r * q1 / prm.nu;
         const double ff = (p1 - p2) / q1 / q1 / 3.0 * 4.0 * r / (double)(o.lz - 2 * o.d2);
         double probe[4];
@@ -306,6 +399,11 @@ int main(int argc, char **argv)
                 std::fclose(io);
             }
         }
+    };
+    for (long c = c0 + 1; c <= c0 + o.steps; c++) {
+        check(fslbm_step(ctx, 1), ctx, "step");
+        pending_gauge = 0.0;
+        if (c % o.monitor == 0) monitor_step(c);
         if (o.backup > 0 && c % o.backup == 0) {
             // FSLBM.cu:825-904: raw float32 padded volumes + text averages
             for (int f = 0; f < 4; f++) {
@@ -321,6 +419,7 @@ int main(int argc, char **argv)
             }
             copy_file("qp.txt", "qpii.txt");
             write_averages(ctx, "ii");
+            write_checkpoint(ctx, nx, ny, nz, o.precision, c, pending_gauge);
         }
     }
     check(fslbm_sync(ctx), ctx, "sync");

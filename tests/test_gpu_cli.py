@@ -115,3 +115,23 @@ def test_cli_backup_and_restart(tmp_path):
     assert qp.shape == (3, 6)
     cnt1 = (tmp_path / "_av.txt").read_text().split()
     assert int(cnt1[0]) == int(cnt0[0]) + 1
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_cli_exact_checkpoint_resumes_bitwise(tmp_path, precision):
+    """A run interrupted at a backup and resumed from fslbm_ckpt.bin writes
+    the same files as an uninterrupted run (state, statistics sums, step
+    number and pending gauge are restored exactly)."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    write_inputs(a)
+    write_inputs(b)
+    common = ["--monitor", 50, "--stats", 50, "--precision", precision]
+    run_cli(a, "--steps", 150, "--backup", 0, "--fresh", *common)
+    run_cli(b, "--steps", 100, "--backup", 100, "--fresh", *common)
+    assert (b / "fslbm_ckpt.bin").exists()
+    run_cli(b, "--steps", 50, "--backup", 0, *common)
+    for name in ("qp.txt", "_p.txt", "_uz.txt", "_uz2.txt", "_uxz2.txt", "_av.txt", "p.txt",
+                 "uz.txt", "uz1.txt"):
+        assert (a / name).read_text() == (b / name).read_text(), name
