@@ -1,6 +1,7 @@
 // Batched domains: many independent FSLBM domains (pipes, periodic
-// boxes) advanced together, one persistent launch per kernel class and
-// step, over a packed table of (domain, work item).  Domains are admitted
+// boxes) advanced together, one launch per kernel class and step, over a
+// packed table of (domain, work item), one workgroup per item.  Domains are
+// admitted
 // against an HBM budget and removed when done; the device state of every
 // admitted domain stays resident.  The launch sequence of a step can be
 // captured in a hipGraph and replayed.
@@ -59,7 +60,6 @@ struct fslbm_batch {
     bool dirty = true;
     int flip = 0;   // steps since the descriptors were written, mod 2
     int chunk = 0;  // schedule (0: default per domain)
-    int grid = 256; // persistent workgroups (one per CU)
     bool use_graph = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps from flip 0 / 1
     hipStream_t stream = nullptr;
@@ -410,10 +410,6 @@ int fslbm_batch_create(int precision, int d1, size_t hbm_budget, fslbm_batch_t *
         fslbm_batch_destroy(b);
         return FSLBM_EHIP;
     }
-    int dev = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-        b->grid = 8 * std::max(1, prop.multiProcessorCount / 8);
     *out = b;
     return FSLBM_OK;
 }

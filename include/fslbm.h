@@ -235,8 +235,9 @@ int fslbm_fold_mask(int lx, int ly, int lz, int d2, const uint8_t *padded,
 /* ------------------------------------------------------------------ */
 /* Batched domains (no counterpart in the reference, which runs one     */
 /* hard-coded pipe): many independent pipes / periodic boxes advanced   */
-/* together, one persistent launch per step over a packed table of      */
-/* (domain, tile x z-chunk) work items.  Every work item runs the same  */
+/* together, one launch per kernel class and step over a packed table  */
+/* of (domain, tile x z-chunk) work items, one workgroup per item.      */
+/* Every work item runs the same                                        */
 /* device code as the single-domain tiled kernel, so a domain's result  */
 /* is bitwise the same alone or in any mix.  Domains are admitted       */
 /* against an HBM budget and stay device-resident until removed.        */
