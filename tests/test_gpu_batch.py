@@ -125,3 +125,21 @@ def test_periodic_partial_tiles_are_deterministic(precision):
         sl = bt.add(prm, st, mask)
         bt.step(60)
         assert all(np.array_equal(g, e) for g, e in zip(bt.get_state(sl), a))
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16s"])
+def test_batch_d1_1_equals_alone(precision):
+    """A batch of d1 = 1 domains (corrector spacing 1) with both kernel
+    classes (blend 0 and blend != 0) matches solo runs bitwise."""
+    cases = [pipe_case(30, 40, 12.5, 300, 0.01, 5),
+             pipe_case(26, 33, 10.0, 200, 0.01, 6, blend=1.0, tb=0.8)]
+    cases = [(RunParams(tblend=p.tblend, pin=p.pin, nu=p.nu, d1=1, delta=1, blend=p.blend), m, s)
+             for p, m, s in cases]
+    ref = [alone(prm, mask, st, 17, precision) for prm, mask, st in cases]
+    with Batch(precision, 1, 8 << 30) as b:
+        b.use_graph(True)
+        slots = [b.add(prm, st, mask) for prm, mask, st in cases]
+        b.step(17)
+        for slot, r in zip(slots, ref):
+            assert all(np.isfinite(e).all() for e in r)
+            assert all(np.array_equal(g, e) for g, e in zip(b.get_state(slot), r))
