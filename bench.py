@@ -10,8 +10,9 @@ One step = one FSLBM time step over all 67,108,864 lattice cells.
 Printed: one JSON line with MLUPS (all lattice cells, the paper's
 convention), the HBM roofline of the step kernel and the CPU baseline
 (the C oracle with OpenMP on a bounded sample of the same workload), with
-sub-objects for the 256^3 Taylor-Green box (FP32 / FP16 storage against
-FP64), the 64 uneven pipes in one batch, and the serving stream.
+sub-objects for the small 32x32x64 FP64 pipe (GPU against the CPU oracle
+at steps 100 and 1000), the 256^3 Taylor-Green box (FP32 / FP16 storage
+against FP64), the 64 uneven pipes in one batch, and the serving stream.
 Other workloads: --workload pipe|tgv|batch|serve|stats.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32]
@@ -382,6 +383,46 @@ def bench_stats(steps: int, window: int, every: int, precisions):
     return out
 
 
+def bench_small_pipe():
+    """The small-pipe case: 32x32x64, radius 14, FP64, bulk Re 200 (nu 0.01),
+    Poiseuille + 1% noise; 1000 steps on the GPU (tiled kernel) and in the
+    C oracle on the host cores, compared at steps 100 and 1000."""
+    from flowcontrast_amd import KERNEL_AUTO, RunParams, Solver, inputs
+    from oracle import oracle as O
+    nx = ny = 32
+    nz, R, nu = 64, 14.0, 0.01
+    ubar = 200 * nu / (2 * R)
+    pin = inputs.laminar_pin(nu, nz, ubar, R)
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    st = inputs.poiseuille_state(mask, R, ubar, pin, 0.01, 1)
+    prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=0.0)
+    oprm = O.params(1.0, pin, nu, 0.0, 2)
+    cells = nx * ny * nz
+    out = {"workload": "pipe32x32x64_r14, FP64, Re 200, 1000 steps", "cells": cells}
+    with Solver(mask.shape, prm, mask=mask, precision="f64", kernel=KERNEL_AUTO) as s:
+        s.set_state(*st)
+        s.step(100)
+        g100 = s.get_state()
+        s.step(900)
+        g1000 = s.get_state()
+        s.set_state(*st)
+        ms = s.time_steps(1000)
+    out["gpu_mlups"] = round(cells * 1000 / ms / 1e3, 1)
+    t0 = time.perf_counter()
+    r100 = O.step_folded(st, oprm, mask=mask, nsteps=100)
+    r1000 = O.step_folded(r100, oprm, mask=mask, nsteps=900)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_mlups"] = round(cells * 1000 / cpu_s / 1e6, 1)
+    out["cpu_threads"] = O.max_threads()
+
+    def maxrel(g, r):
+        return max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(g, r))
+
+    out["max_rel_diff_step100"] = float(f"{maxrel(g100, r100):.3e}")
+    out["max_rel_diff_step1000"] = float(f"{maxrel(g1000, r1000):.3e}")
+    return out
+
+
 def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
     """Taylor-Green vortex on an n^3 periodic box (U0 = 0.05, Re 1600):
     step rate and roofline of the unmasked kernel, and the kinetic-energy
@@ -565,7 +606,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(mask, st, pin, nu, args.cpu_steps,
                                            args.precision)
     if args.workload == "all":
-        for key, fn in (("taylor_green", lambda: bench_tgv(256, 5000, 250, 100,
+        for key, fn in (("small_pipe_f64", bench_small_pipe),
+                        ("taylor_green", lambda: bench_tgv(256, 5000, 250, 100,
                                                           ["f64", "f32", "f16s"])),
                         ("batched", lambda: bench_batch(args.batch_count, args.batch_steps,
                                                        not args.no_sequential)),
