@@ -196,7 +196,11 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
     cases = W.serving_requests(count, 5, rate)
     built = [c.build() for c in cases]
     work = sum(c.cells * c.steps for c in cases)
-    spot_ids = set(sorted(range(count), key=lambda k: cases[k].cells * cases[k].steps)[:spot])
+    # spot checks: the smallest requests of each precision mode
+    spot_ids = set()
+    for prec in ("f32", "f16s"):
+        ks = [k for k in range(count) if cases[k].precision == prec]
+        spot_ids |= set(sorted(ks, key=lambda k: cases[k].cells * cases[k].steps)[:spot])
     print(f"bench: serving {count} requests at {rate}/s", file=sys.stderr, flush=True)
 
     def run(open_loop: bool):
@@ -491,7 +495,8 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-quantum", type=int, default=200)
     ap.add_argument("--serve-budget", type=int, default=32 << 30)
-    ap.add_argument("--spot-checks", type=int, default=3)
+    ap.add_argument("--spot-checks", type=int, default=2,
+                    help="serving: requests of each precision checked against FP64")
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--stats-steps", type=int, default=50000)
     ap.add_argument("--stats-window", type=int, default=20000)
