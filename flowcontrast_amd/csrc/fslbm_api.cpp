@@ -38,6 +38,9 @@ struct fslbm_ctx {
     bool z_invariant = false;  // every z plane of the mask is the same
     uint8_t *mask2d = nullptr; // that plane (device), for the tiled kernel
     void *omega = nullptr;     // per-column wall weights (device)
+    int *tiles = nullptr;      // tiles holding fluid (device; nullptr: all)
+    int ntiles = 0;
+    long long tile_cells = 0;  // cells of those tiles
     int chunk = 0;             // tiled kernel z planes per work item
     double *partials = nullptr;  // reduction scratch
     int npart = 1024;
@@ -189,7 +192,7 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     if (c->kernel == FSLBM_KERNEL_TILED && !tiled_ok(c, c->prm.d1))
         return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
     if (use_tiled(c)) {
-        TiledAux aux{c->mask2d, c->omega, c->chunk, c->stream};
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, c->chunk, c->stream};
         e = launch_step_tiled(c->g, c->precision, in, out, aux, a);
     } else
         e = launch_step_plain(c->g, c->precision, in, out, c->mask, a, c->stream);
@@ -234,6 +237,7 @@ void fslbm_destroy(fslbm_t c)
     if (c->mask) (void)hipFree(c->mask);
     if (c->mask2d) (void)hipFree(c->mask2d);
     if (c->omega) (void)hipFree(c->omega);
+    if (c->tiles) (void)hipFree(c->tiles);
     if (c->partials) (void)hipFree(c->partials);
     if (c->red) (void)hipFree(c->red);
     if (c->hred) (void)hipHostFree(c->hred);
@@ -344,6 +348,14 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
                      hipMemcpy(c->omega, om.data(), om.size() * sizeof(float),
                                hipMemcpyHostToDevice) == hipSuccess;
             }
+            const std::vector<int> tl = active_tiles(c->g, precision, mask);
+            if (ok && !tl.empty()) {
+                ok = alloc((void **)&c->tiles, tl.size() * sizeof(int)) &&
+                     hipMemcpy(c->tiles, tl.data(), tl.size() * sizeof(int),
+                               hipMemcpyHostToDevice) == hipSuccess;
+                c->ntiles = (int)tl.size();
+                c->tile_cells = tiles_cells(c->g, precision, tl);
+            }
         }
     }
     if (ok) {
@@ -420,13 +432,15 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
     a.d1 = c->prm.d1;
     a.q = c->q;
     a.gauge = 0.0;
-    const int cands[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    // the modelled best chunk lengths, then a few fixed ones
+    std::vector<int> cands = chunk_candidates(c->g, c->precision, c->ntiles, 6);
+    for (int f : {32, 64, 128, 256})
+        if (f <= c->g.nz && std::find(cands.begin(), cands.end(), f) == cands.end())
+            cands.push_back(f);
     int best = 0;
     double bms = 1e300;
-    const int keep = c->chunk;
     for (int cand : cands) {
-        if (cand > c->g.nz && cand != cands[0]) break;
-        TiledAux aux{c->mask2d, c->omega, std::min(cand, c->g.nz), c->stream};
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, std::min(cand, c->g.nz), c->stream};
         // warm-up launch, then timed launches
         HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -439,7 +453,6 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
         const double per = ms / trial_steps;
         if (per < bms) { bms = per; best = std::min(cand, c->g.nz); }
     }
-    (void)keep;
     c->chunk = best;
     if (best_chunk) *best_chunk = best;
     if (best_ms) *best_ms = bms;
@@ -449,7 +462,14 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
 int fslbm_get_tile_chunk(fslbm_t c)
 {
     if (!c) return FSLBM_EINVAL;
-    return c->chunk > 0 ? std::min(c->chunk, c->g.nz) : default_chunk(c->g, c->precision);
+    return c->chunk > 0 ? std::min(c->chunk, c->g.nz)
+                        : default_chunk(c->g, c->precision, c->ntiles);
+}
+
+long long fslbm_step_cells(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    return use_tiled(c) && c->ntiles > 0 ? c->tile_cells : c->g.N;
 }
 
 int fslbm_get_kernel(fslbm_t c)

@@ -31,6 +31,8 @@ struct Domain {
     void *buf[2] = {nullptr, nullptr};
     uint8_t *mask2d = nullptr;
     void *omega = nullptr;
+    int *tiles = nullptr;  // tiles holding fluid (nullptr: all)
+    int ntiles = 0;
     std::vector<uint8_t> hmask2d;
     size_t bytes = 0;
     int parity = 0;  // buffer holding the current state
@@ -127,7 +129,8 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         d.parity ^= b->flip;
         const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
         const int nw = batch_fill_dom(host.data() + i * db, b->precision, d.g, step_args(d), ch,
-                                      d.buf[0], d.buf[1], d.mask2d, d.omega, d.parity);
+                                      d.buf[0], d.buf[1], d.mask2d, d.omega, d.tiles,
+                                      d.ntiles, d.parity);
         if (nw < 0) return bfail(b, FSLBM_EINVAL, "domain %zu: unsupported precision", i);
         seeds[2 * i] = (uint32_t)d.steps;  // the next launch has flip 0
         seeds[2 * i + 1] = (uint32_t)d.steps + 1u;
@@ -246,7 +249,9 @@ size_t domain_bytes(int precision, int nx, int ny, int nz, bool masked)
 {
     const size_t N = (size_t)nx * ny * nz, nxy = (size_t)nx * ny;
     const size_t rb = precision == FSLBM_F64 ? 8 : 4;
-    return 2 * 4 * N * storage_bytes(precision) + (masked ? nxy + 3 * nxy * rb : 0);
+    const size_t tiles = (size_t)((nx + 63) / 64) * ((ny + 7) / 8);  // bound on the tile list
+    return 2 * 4 * N * storage_bytes(precision) +
+           (masked ? nxy + 3 * nxy * rb + tiles * sizeof(int) : 0);
 }
 
 }  // namespace
@@ -300,7 +305,10 @@ int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, 
     if (b->precision != FSLBM_F32) choose_quant(g, hm, prm->pin, src, pd.q);
     pd.host.resize(4 * (size_t)N * storage_bytes(b->precision));
     encode_state(g, hm, b->precision, pd.q, src, pd.host.data());
-    if (mask) pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
+    if (mask) {
+        pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
+        pd.tiles = active_tiles(g, b->precision, pd.mask2d.data());
+    }
     return FSLBM_OK;
 }
 
@@ -330,6 +338,7 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
         for (void *q : d.buf) if (q) (void)hipFree(q);
         if (d.mask2d) (void)hipFree(d.mask2d);
         if (d.omega) (void)hipFree(d.omega);
+        if (d.tiles) (void)hipFree(d.tiles);
     };
     bool ok = hipMalloc(&d.buf[0], sbytes) == hipSuccess &&
               hipMalloc(&d.buf[1], sbytes) == hipSuccess;
@@ -338,6 +347,12 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
              hipMalloc(&d.omega, pd.omega.size() * sizeof(float)) == hipSuccess &&
              hipMemcpyAsync(d.mask2d, d.hmask2d.data(), nxy, hipMemcpyHostToDevice, s) == hipSuccess &&
              hipMemcpyAsync(d.omega, pd.omega.data(), pd.omega.size() * sizeof(float), hipMemcpyHostToDevice, s) == hipSuccess;
+        if (ok && !pd.tiles.empty()) {
+            ok = hipMalloc((void **)&d.tiles, pd.tiles.size() * sizeof(int)) == hipSuccess &&
+                 hipMemcpyAsync(d.tiles, pd.tiles.data(), pd.tiles.size() * sizeof(int),
+                                hipMemcpyHostToDevice, s) == hipSuccess;
+            d.ntiles = (int)pd.tiles.size();
+        }
     }
     if (ok)
         ok = hipMemcpyAsync(d.buf[0], pd.host.data(), sbytes, hipMemcpyHostToDevice, s) == hipSuccess &&
@@ -350,6 +365,7 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
     if (sync) {
         pd.host = std::vector<unsigned char>();
         pd.omega = std::vector<float>();
+        pd.tiles = std::vector<int>();
     }  // else the caller keeps them until it has synchronised s
     // reuse a free slot
     int slot = -1;
@@ -423,6 +439,7 @@ void fslbm_batch_destroy(fslbm_batch_t b)
         for (void *p : d.buf) if (p) (void)hipFree(p);
         if (d.mask2d) (void)hipFree(d.mask2d);
         if (d.omega) (void)hipFree(d.omega);
+        if (d.tiles) (void)hipFree(d.tiles);
     }
     if (b->d_doms) (void)hipFree(b->d_doms);
     if (b->d_seeds) (void)hipFree(b->d_seeds);
@@ -470,8 +487,10 @@ int fslbm_batch_remove(fslbm_batch_t b, int slot)
     for (void *&q : d.buf) if (q) { (void)hipFree(q); q = nullptr; }
     if (d.mask2d) (void)hipFree(d.mask2d);
     if (d.omega) (void)hipFree(d.omega);
+    if (d.tiles) (void)hipFree(d.tiles);
     d.mask2d = nullptr;
     d.omega = nullptr;
+    d.tiles = nullptr;
     b->used -= d.bytes;
     d = Domain();
     b->dirty = true;

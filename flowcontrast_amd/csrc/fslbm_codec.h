@@ -240,6 +240,7 @@ struct PreparedDomain {
     std::vector<uint8_t> mask2d;      // one z plane; empty for a periodic box
     std::vector<unsigned char> host;  // encoded state, 4 planes
     std::vector<float> omega;         // wall weights (walled domains)
+    std::vector<int> tiles;           // tiles holding fluid (walled domains)
     size_t bytes = 0;                 // device bytes in the batch
 };
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/fslbm.h"
 
 namespace fslbm {
@@ -56,12 +58,16 @@ size_t storage_bytes(int precision);  // bytes per stored value
 hipError_t launch_step_plain(const Geo &g, int precision, const void *in,
                              void *out, const uint8_t *mask,
                              const StepArgs &a, hipStream_t s);
-// Extra inputs of the tiled kernel: the z-invariant mask as one plane and
+// Extra inputs of the tiled kernel: the z-invariant mask as one plane,
 // the per-column wall weights omega (3 planes of nx*ny values in the
-// arithmetic type: sum over wall neighbours of W_e, W_e cx, W_e cy).
+// arithmetic type: sum over wall neighbours of W_e, W_e cx, W_e cy) and
+// the device list of tiles that hold a fluid cell (from active_tiles;
+// null: every tile).
 struct TiledAux {
     const uint8_t *mask2d;
     const void *omega;
+    const int *tiles;
+    int ntiles;
     int chunk;  // z planes per work item (0: default)
     hipStream_t stream;
 };
@@ -69,7 +75,18 @@ hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
                              void *out, const TiledAux &aux,
                              const StepArgs &a);
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
-int default_chunk(const Geo &g, int precision);
+// Compute units of the current device.
+int device_cus();
+// Tile indices (ty * ntx + tx, in the tile shape of the precision mode)
+// whose cells include a fluid cell of the z-invariant mask plane; empty for
+// a periodic box (every tile).
+std::vector<int> active_tiles(const Geo &g, int precision, const uint8_t *mask2d);
+// Lattice cells (all z planes) covered by those tiles, clipped to the box.
+long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles);
+// Chunk lengths (z planes per work item) ordered by the modelled step
+// time over `ntiles` tiles (0: every tile); default_chunk is the first.
+std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count);
+int default_chunk(const Geo &g, int precision, int ntiles);
 
 // Batched step over many domains (32-bit arithmetic modes).  `doms` is a
 // device array of per-domain descriptors (batch_dom_bytes each, filled by
@@ -90,7 +107,7 @@ size_t batch_dom_bytes(int precision);
 // returns the domain's number of work items
 int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
-                   int parity);
+                   const int *tiles, int ntiles, int parity);
 void batch_set_parity(void *dst, int precision, int parity);
 struct HostResult;      // fslbm_codec.h
 struct PreparedDomain;  // fslbm_codec.h

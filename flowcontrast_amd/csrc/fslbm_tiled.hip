@@ -30,11 +30,19 @@
 // * Workgroups are mapped XCD by XCD: the 8 XCDs each take a contiguous
 //   run of work items (z-chunk major), so tiles that share halos share
 //   an L2.
+// * Work items run over the tiles that hold a fluid cell only: a tile of
+//   wall cells would read zeros and store nothing (walls are never
+//   written, and both state buffers hold zeros there).  The z planes split
+//   into chunks whose lengths differ by at most one, and the chunk count
+//   is chosen so that the items fill whole rounds of workgroups.
 //
 // Roles of the 704 threads (11 waves):
 //   waves 0-7  interior columns (64 x 8): everything
 //   waves 8-9  halo rows y0-1 and y0+8: z pass, x pass, y messages
 //   wave 10    halo columns x0-1 and x0+64 (20 lanes): z pass, x messages
+
+#include <algorithm>
+#include <utility>
 
 #include "fslbm_internal.h"
 #include "fslbm_store.h"
@@ -153,6 +161,11 @@ template <typename R>
 struct TiledParams {
     int nx, ny, nz, nxy;
     int ntx, nty, chunk, nchunks, nwork, per_xcd;
+    // work items: chunk-major over ntile tiles (indices ty*ntx + tx from
+    // `tiles`, or all tiles in order when it is null); chunk c covers
+    // cq + (c < cr) planes from c*cq + min(c, cr)
+    int ntile, cq, cr;
+    const int *tiles;
     R pin, gauge;
     R c0, c1, c2, g, tb, cc;
     // blend folded into the corrector and the 1/216 scale of the
@@ -161,6 +174,18 @@ struct TiledParams {
     Quant q;
     uint32_t skey;  // stochastic-rounding key of this step (16-bit modes)
 };
+
+// Tile index and z range [zs, ze) of work item w.
+template <typename R>
+__device__ __forceinline__ void item_of(const TiledParams<R> &P, int w, int &tile, int &zs,
+                                        int &ze)
+{
+    const int c = w / P.ntile;
+    const int slot = w - c * P.ntile;
+    tile = P.tiles ? P.tiles[slot] : slot;
+    zs = c * P.cq + min(c, P.cr);
+    ze = zs + P.cq + (c < P.cr ? 1 : 0);
+}
 
 __device__ __forceinline__ int wrapi(int v, int n)
 {
@@ -251,13 +276,10 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
     const int bid = blockIdx.x;
     const int w = (bid & 7) * P.per_xcd + (bid >> 3);
     if (w >= P.nwork) return;
-    const int ntile = P.ntx * P.nty;
-    const int chunk = w / ntile;
-    const int tile = w - chunk * ntile;
+    int tile, zs, ze;
+    item_of(P, w, tile, zs, ze);
     const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
     const int x0 = tx * TX, y0 = ty * TY;
-    const int zs = chunk * P.chunk;
-    const int ze = min(zs + P.chunk, P.nz);
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
     const int N = nxy * nz;
 
@@ -586,13 +608,10 @@ __device__ __forceinline__ void pipe_item(
     __shared__ FdCell<R> fd[2][FY][FX];
     __shared__ R edge[2][RYP][2][8];  // [side 0: column x0-1 | 1: x0+64]
 
-    const int ntile = P.ntx * P.nty;
-    const int chunk = w / ntile;
-    const int tile = w - chunk * ntile;
+    int tile, zs, ze;
+    item_of(P, w, tile, zs, ze);
     const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
     const int x0 = tx * TX, y0 = ty * TYP;
-    const int zs = chunk * P.chunk;
-    const int ze = min(zs + P.chunk, P.nz);
     const int n = ze - zs;
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
     const int N = nxy * nz;
@@ -1110,7 +1129,7 @@ hipError_t launch_batch_mode(const BatchLaunch &b)
 // Host-side fill of the per-launch parameters of one domain.
 template <int MODE>
 void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const StepArgs &a,
-                 int chunk_req)
+                 int chunk_req, const int *tiles, int ntiles)
 {
     using R = typename Store<MODE>::R;
     const StepK<R> k = [&] {
@@ -1124,10 +1143,14 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.nxy = g.nx * g.ny;
     P.ntx = (g.nx + TX - 1) / TX;
     P.nty = (g.ny + rows - 1) / rows;
-    P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, MODE);
+    P.tiles = tiles && ntiles > 0 ? tiles : nullptr;
+    P.ntile = P.tiles ? ntiles : P.ntx * P.nty;
+    P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, MODE, P.ntile);
     if (P.chunk > g.nz) P.chunk = g.nz;
     P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
-    P.nwork = P.ntx * P.nty * P.nchunks;
+    P.cq = g.nz / P.nchunks;
+    P.cr = g.nz - P.cq * P.nchunks;
+    P.nwork = P.ntile * P.nchunks;
     P.per_xcd = (P.nwork + 7) / 8;
     P.pin = k.pin;
     P.gauge = k.gauge;
@@ -1176,7 +1199,7 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
 {
     using R = typename Store<MODE>::R;
     TiledParams<R> P;
-    fill_params<MODE>(P, g, a, aux.chunk);
+    fill_params<MODE>(P, g, a, aux.chunk, aux.tiles, aux.ntiles);
     const bool per = g.periodic != 0;
     if (a.d1 == 1)
         return per ? launch_t<MODE, true, 1>(g, in, out, aux, P, aux.stream)
@@ -1187,15 +1210,107 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
 
 }  // namespace
 
-int default_chunk(const Geo &g, int precision)
+int device_cus()
 {
-    // About three work items per CU (256 CUs), chunks of >= 16 planes.
-    const int rows = precision == FSLBM_F64 ? TY : pipe_rows(precision);
-    const long long tiles = (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
-    long long c = (g.nz * tiles + 767) / 768;
-    if (c < 16) c = 16;
-    if (c > g.nz) c = g.nz;
-    return (int)c;
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] <= 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            v <= 0)
+            v = 256;
+        cached[dev] = v;
+    }
+    return cached[dev];
+}
+
+namespace {
+
+// Cost of starting a work item (pipeline fill, first loads, dispatch) in
+// units of one plane of a tile.  With 3 the model ranks the measured step
+// times of the 256x256x1024 pipe (78 or 112 tiles) over chunk lengths 64
+// to 342 in their measured order.
+constexpr int ITEM_OVERHEAD = 3;
+
+// Modelled duration of a step with `nch` chunks per tile: one workgroup
+// per CU at a time, each XCD running its contiguous run of items in
+// rounds.
+double chunk_cost(int nz, long long ntile, int nch, int cus)
+{
+    const long long items = ntile * nch;
+    const long long per_xcd = (items + 7) / 8;
+    const long long slots = std::max(1, cus / 8);
+    const long long rounds = (per_xcd + slots - 1) / slots;
+    const int len = (nz + nch - 1) / nch;
+    return (double)rounds * (len + ITEM_OVERHEAD);
+}
+
+int tile_rows(int precision) { return precision == FSLBM_F64 ? TY : pipe_rows(precision); }
+
+}  // namespace
+
+std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count)
+{
+    const int rows = tile_rows(precision);
+    const long long T = ntiles > 0 ? ntiles
+                                   : (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
+    const int cus = device_cus();
+    const int minlen = std::min(16, g.nz);
+    std::vector<std::pair<double, int>> c;  // (cost, chunk length)
+    for (int nch = 1; nch <= g.nz / minlen; nch++) {
+        const int len = (g.nz + nch - 1) / nch;
+        const int real = (g.nz + len - 1) / len;  // chunks of that length
+        if (real != nch) continue;
+        c.push_back({chunk_cost(g.nz, T, nch, cus), len});
+    }
+    if (c.empty()) c.push_back({0.0, g.nz});
+    // cheapest first; among equal costs the longer chunk (fewer items)
+    std::sort(c.begin(), c.end(), [](const auto &a, const auto &b) {
+        return a.first < b.first || (a.first == b.first && a.second > b.second);
+    });
+    std::vector<int> out;
+    for (const auto &e : c) {
+        if ((int)out.size() >= count) break;
+        out.push_back(e.second);
+    }
+    return out;
+}
+
+int default_chunk(const Geo &g, int precision, int ntiles)
+{
+    return chunk_candidates(g, precision, ntiles, 1)[0];
+}
+
+std::vector<int> active_tiles(const Geo &g, int precision, const uint8_t *mask2d)
+{
+    std::vector<int> out;
+    if (g.periodic || !mask2d) return out;
+    const int rows = tile_rows(precision);
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + rows - 1) / rows;
+    for (int ty = 0; ty < nty; ty++)
+        for (int tx = 0; tx < ntx; tx++) {
+            bool fluid = false;
+            for (int y = ty * rows; y < std::min(g.ny, (ty + 1) * rows) && !fluid; y++)
+                for (int x = tx * TX; x < std::min(g.nx, (tx + 1) * TX); x++)
+                    if (mask2d[(size_t)y * g.nx + x] == 0) { fluid = true; break; }
+            if (fluid) out.push_back(ty * ntx + tx);
+        }
+    // a mask without fluid still gets one item, which stores nothing
+    if (out.empty()) out.push_back(0);
+    return out;
+}
+
+long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles)
+{
+    const int rows = tile_rows(precision);
+    const int ntx = (g.nx + TX - 1) / TX;
+    long long n = 0;
+    for (int t : tiles) {
+        const int ty = t / ntx, tx = t - ty * ntx;
+        n += (long long)std::min(TX, g.nx - tx * TX) * std::min(rows, g.ny - ty * rows);
+    }
+    return n * g.nz;
 }
 
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant)
@@ -1228,11 +1343,11 @@ size_t batch_dom_bytes(int precision)
 
 int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
-                   int parity)
+                   const int *tiles, int ntiles, int parity)
 {
     auto fill = [&](auto *D, auto tag) -> int {
         constexpr int M = decltype(tag)::value;
-        fill_params<M>(D->P, g, a, chunk);
+        fill_params<M>(D->P, g, a, chunk, tiles, ntiles);
         D->buf[0] = buf0;
         D->buf[1] = buf1;
         D->mask2d = mask2d;

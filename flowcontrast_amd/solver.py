@@ -106,6 +106,12 @@ class Solver:
     def tile_chunk(self) -> int:
         return self._chk(lib().fslbm_get_tile_chunk(self._h))
 
+    @property
+    def step_cells(self) -> int:
+        """Lattice cells one step reads and writes (the tiled kernel skips
+        tiles of wall cells)."""
+        return self._chk(lib().fslbm_step_cells(self._h))
+
     def autotune(self, trial_steps: int = 5):
         """Pick the fastest tiled schedule; returns (chunk, ms per step).
         The state is not advanced."""

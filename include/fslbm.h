@@ -103,6 +103,13 @@ int fslbm_get_kernel(fslbm_t ctx);
 int fslbm_set_tile_chunk(fslbm_t ctx, int chunk);
 int fslbm_get_tile_chunk(fslbm_t ctx);
 
+/* Lattice cells one step reads and writes (no reference counterpart; for
+ * bandwidth accounting).  The tiled kernel skips tiles that hold only wall
+ * cells (their state stays zero), so this is the cell count of the tiles
+ * holding fluid; the plain kernel covers all nx*ny*nz cells.  Returns a
+ * negative error code on failure. */
+long long fslbm_step_cells(fslbm_t ctx);
+
 /* Autotuner of the tiled kernel's schedule: times trial_steps launches for
  * each candidate chunk (HIP events on the context's stream), keeps the
  * fastest and reports it (*best_chunk, *best_ms per step; either may be
