@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <queue>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,7 @@ struct Domain {
     void *omega = nullptr;
     int *tiles = nullptr;  // tiles holding fluid (nullptr: all)
     int ntiles = 0;
+    std::vector<int> htiles;  // host copy (empty: all tiles)
     std::vector<uint8_t> hmask2d;
     size_t bytes = 0;
     int parity = 0;  // buffer holding the current state
@@ -56,7 +58,7 @@ struct fslbm_batch {
     size_t d_doms_cap = 0;
     uint32_t *d_seeds = nullptr;  // two step counters per domain slot
     size_t d_seeds_cap = 0;
-    int2 *d_work[NCLASS] = {nullptr, nullptr, nullptr, nullptr};
+    int4 *d_work[NCLASS] = {nullptr, nullptr, nullptr, nullptr};
     size_t work_cap[NCLASS] = {0, 0, 0, 0};
     int nwork[NCLASS] = {0, 0, 0, 0};
     bool dirty = true;
@@ -114,29 +116,95 @@ void drop_graphs(fslbm_batch *b)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
+// One z column of one tile of a domain, still to be cut into work items.
+struct Column {
+    int dom, tile, z0, rem;
+};
+
+// Guided static split of the columns of one kernel class into work items.
+// Each cut takes L = (remaining planes of the class) / (2 * CUs) planes,
+// clamped to [lmin, lmax], from the column with the most planes left, so
+// item lengths shrink as the table fills and the last items are short.
+// The table is ordered longest first; workgroup b takes item b and the
+// dispatcher deals workgroups to XCD b mod 8, so each XCD gets an even
+// share and the step's tail is one short item.  One item per domain
+// carries BATCH_LEAD.  The cut changes only the schedule: every cell is
+// computed the same way whatever item holds it.
+std::vector<int4> guided_split(std::vector<Column> &cols, int lmax, int cus, size_t ndom)
+{
+    const int lmin = std::min(8, lmax);
+    long long left = 0;
+    for (const Column &c : cols) left += c.rem;
+    auto fewer = [&](int a, int b) {
+        return cols[a].rem < cols[b].rem || (cols[a].rem == cols[b].rem && a > b);
+    };
+    std::priority_queue<int, std::vector<int>, decltype(fewer)> pq(fewer);
+    for (int k = 0; k < (int)cols.size(); k++)
+        if (cols[k].rem > 0) pq.push(k);
+    std::vector<std::pair<int, int4>> items;
+    while (!pq.empty()) {
+        const int k = pq.top();
+        pq.pop();
+        Column &c = cols[k];
+        long long L = (left + 2LL * cus - 1) / (2LL * cus);
+        L = std::max<long long>(lmin, std::min<long long>(lmax, L));
+        int take = (int)std::min<long long>(L, c.rem);
+        if (c.rem - take < lmin)  // no sliver at the column end
+            take = c.rem <= lmax ? c.rem : c.rem / 2;
+        items.push_back({take, make_int4(c.dom, c.tile, c.z0, c.z0 + take)});
+        c.z0 += take;
+        c.rem -= take;
+        left -= take;
+        if (c.rem > 0) pq.push(k);
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    std::vector<int4> out;
+    out.reserve(items.size());
+    std::vector<char> led(ndom, 0);
+    for (const auto &e : items) {
+        int4 it = e.second;
+        if (!led[it.x]) {
+            led[it.x] = 1;
+            it.w |= BATCH_LEAD;
+        }
+        out.push_back(it);
+    }
+    return out;
+}
+
 // Rewrite the descriptors and work tables (after add/remove/retune).
+// chunk_override >= 0 replaces the batch's chunk (the longest item, in z
+// planes; 0: the default of 128).
 int rebuild(fslbm_batch *b, int chunk_override = -1)
 {
     const size_t db = batch_dom_bytes(b->precision);
     const size_t nd = b->doms.size();
     std::vector<unsigned char> host(std::max<size_t>(nd, 1) * db, 0);
     std::vector<uint32_t> seeds(2 * std::max<size_t>(nd, 1), 0u);
-    std::vector<int2> work[NCLASS];
+    const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
+    std::vector<Column> cols[NCLASS];
     for (size_t i = 0; i < nd; i++) {
         Domain &d = b->doms[i];
         if (!d.active || d.held) continue;
         // the current parity of each domain is folded into its descriptor
         d.parity ^= b->flip;
-        const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
-        const int nw = batch_fill_dom(host.data() + i * db, b->precision, d.g, step_args(d), ch,
+        int split[3] = {1, 0, 0};
+        const int nw = batch_fill_dom(host.data() + i * db, b->precision, d.g, step_args(d), 0,
                                       d.buf[0], d.buf[1], d.mask2d, d.omega, d.tiles,
-                                      d.ntiles, d.parity);
+                                      d.ntiles, d.parity, split);
         if (nw < 0) return bfail(b, FSLBM_EINVAL, "domain %zu: unsupported precision", i);
         seeds[2 * i] = (uint32_t)d.steps;  // the next launch has flip 0
         seeds[2 * i + 1] = (uint32_t)d.steps + 1u;
-        auto &wl = work[class_of(d)];
-        for (int w = 0; w < nw; w++) wl.push_back(make_int2((int)i, w));
+        auto &cl = cols[class_of(d)];
+        if (d.htiles.empty())
+            for (int t = 0; t < split[0]; t++) cl.push_back({(int)i, t, 0, d.g.nz});
+        else
+            for (int t : d.htiles) cl.push_back({(int)i, t, 0, d.g.nz});
     }
+    const int cus = device_cus();
+    std::vector<int4> work[NCLASS];
+    for (int c = 0; c < NCLASS; c++) work[c] = guided_split(cols[c], ch > 0 ? ch : 128, cus, nd);
     b->flip = 0;
     if (host.size() > b->d_doms_cap) {
         if (b->d_doms) (void)hipFree(b->d_doms);
@@ -156,7 +224,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     }
     BCHK(b, hipMemcpyAsync(b->d_seeds, seeds.data(), sb, hipMemcpyHostToDevice, b->stream));
     for (int c = 0; c < NCLASS; c++) {
-        const size_t need = std::max<size_t>(work[c].size(), 1) * sizeof(int2);
+        const size_t need = std::max<size_t>(work[c].size(), 1) * sizeof(int4);
         if (need > b->work_cap[c]) {
             if (b->d_work[c]) (void)hipFree(b->d_work[c]);
             b->d_work[c] = nullptr;
@@ -165,7 +233,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
             b->work_cap[c] = need;
         }
         if (!work[c].empty())
-            BCHK(b, hipMemcpyAsync(b->d_work[c], work[c].data(), work[c].size() * sizeof(int2),
+            BCHK(b, hipMemcpyAsync(b->d_work[c], work[c].data(), work[c].size() * sizeof(int4),
                                    hipMemcpyHostToDevice, b->stream));
         b->nwork[c] = (int)work[c].size();
     }
@@ -186,7 +254,7 @@ hipError_t launch_step_all(fslbm_batch *b, int flip)
         L.work = b->d_work[c];
         L.nwork = b->nwork[c];
         L.flip = flip;
-        L.grid = 8 * ((b->nwork[c] + 7) / 8);  // one workgroup per item
+        L.grid = b->nwork[c];  // one workgroup per item
         L.periodic = (c & 2) != 0;
         L.fd2 = (c & 1) != 0;
         L.d1 = b->d1;
@@ -333,6 +401,7 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
     d.q = pd.q;
     d.bytes = bytes;
     d.hmask2d = std::move(pd.mask2d);
+    d.htiles = pd.tiles;
     const size_t sbytes = pd.host.size();
     auto cleanup = [&]() {
         for (void *q : d.buf) if (q) (void)hipFree(q);

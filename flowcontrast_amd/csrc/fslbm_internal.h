@@ -88,6 +88,10 @@ long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles
 std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count);
 int default_chunk(const Geo &g, int precision, int ntiles);
 
+// Flag on a batch work item's ze: the domain's lead item, whose workgroup
+// advances the domain's step counter.
+constexpr int BATCH_LEAD = 0x40000000;
+
 // Batched step over many domains (32-bit arithmetic modes).  `doms` is a
 // device array of per-domain descriptors (batch_dom_bytes each, filled by
 // batch_fill_dom), `work` the packed (domain, work item) table of one
@@ -96,7 +100,7 @@ int default_chunk(const Geo &g, int precision, int ntiles);
 struct BatchLaunch {
     const void *doms;
     uint32_t *seeds;  // two step counters per domain (see k_step_batch)
-    const int2 *work;
+    const int4 *work;  // (domain, tile, zs, ze | BATCH_LEAD)
     int nwork, flip, grid;
     bool periodic, fd2;
     int d1, precision;
@@ -104,10 +108,12 @@ struct BatchLaunch {
 };
 hipError_t launch_step_batch(const BatchLaunch &b);
 size_t batch_dom_bytes(int precision);
-// returns the domain's number of work items
+// Returns the domain's number of work items; `split` (if not null)
+// receives the item layout: tiles per chunk, and chunk c covering
+// split[1] + (c < split[2]) planes.
 int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
-                   const int *tiles, int ntiles, int parity);
+                   const int *tiles, int ntiles, int parity, int split[3]);
 void batch_set_parity(void *dst, int precision, int parity);
 struct HostResult;      // fslbm_codec.h
 struct PreparedDomain;  // fslbm_codec.h

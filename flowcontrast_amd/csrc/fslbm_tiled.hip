@@ -570,16 +570,17 @@ __device__ __forceinline__ float lane_shift(float src, float edge)
         __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
 }
 
-// One work item (tile x chunk) of the pipelined step.  Called by the
-// single-domain kernel and by the batched kernel; all threads of the
-// workgroup call it with the same w.
+// One work item of the pipelined step: tile `tile` (ty * ntx + tx) over
+// the z planes [zs, ze).  Called by the single-domain kernel and by the
+// batched kernel; all threads of the workgroup pass the same item.
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
 __device__ __forceinline__ void pipe_item(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
     const typename Store<MODE>::R *__restrict__ omega,
-    const TiledParams<typename Store<MODE>::R> &P, const int w, const uint32_t skey)
+    const TiledParams<typename Store<MODE>::R> &P, const int tile, const int zs,
+    const int ze, const uint32_t skey)
 {
     using ST = Store<MODE>;
     using R = typename ST::R;
@@ -608,8 +609,6 @@ __device__ __forceinline__ void pipe_item(
     __shared__ FdCell<R> fd[2][FY][FX];
     __shared__ R edge[2][RYP][2][8];  // [side 0: column x0-1 | 1: x0+64]
 
-    int tile, zs, ze;
-    item_of(P, w, tile, zs, ze);
     const int ty = tile / P.ntx, tx = tile - ty * P.ntx;
     const int x0 = tx * TX, y0 = ty * TYP;
     const int n = ze - zs;
@@ -1060,15 +1059,19 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const int bid = blockIdx.x;
     const int w = (bid & 7) * P.per_xcd + (bid >> 3);
     if (w >= P.nwork) return;
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, w, P.skey);
+    int tile, zs, ze;
+    item_of(P, w, tile, zs, ze);
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, tile, zs, ze, P.skey);
 }
 
 // ---------------------------------------------------------------------
 // Batched step: many independent domains advanced by one launch over a
-// packed table of work items (domain, tile x chunk), one workgroup per
-// item, in XCD-major order.  Every item runs pipe_item, the same code as
-// the single-domain kernel, so a domain's result does not depend on what
-// else is in the batch.
+// packed table of work items (domain, tile, z range), one workgroup per
+// item.  The host orders the table longest item first; workgroup b takes
+// item b, and the dispatcher deals workgroups to the 8 XCDs round robin,
+// so every XCD gets an even share and runs its items longest first.
+// Every item runs pipe_item, the same code as the single-domain kernel,
+// so a domain's result does not depend on what else is in the batch.
 // ---------------------------------------------------------------------
 template <typename R>
 struct BatchDom {
@@ -1082,27 +1085,26 @@ struct BatchDom {
 
 // Step counters (stochastic rounding) live beside the read-only
 // descriptors, two per domain: the launch with flip f reads seeds[2d + f];
-// the workgroup of the domain's first work item writes seeds[2d + (f ^ 1)]
-// = that + 1 for the next launch (no launch reads and writes the same
-// slot), so hipGraph replays of the two launches keep counting.
+// the workgroup of the domain's lead item (one per domain, flagged in the
+// table) writes seeds[2d + (f ^ 1)] = that + 1 for the next launch (no
+// launch reads and writes the same slot), so hipGraph replays of the two
+// launches keep counting.
 template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
 __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const BatchDom<typename Store<MODE>::R> *__restrict__ doms,
-    uint32_t *__restrict__ seeds, const int2 *__restrict__ work, int nwork, int flip)
+    uint32_t *__restrict__ seeds, const int4 *__restrict__ work, int nwork, int flip)
 {
     using S = typename Store<MODE>::S;
-    // XCD-major: XCD x takes items [x*per, (x+1)*per)
-    const int per = (nwork + 7) >> 3;
-    const int bid = blockIdx.x;
-    const int w = (bid & 7) * per + (bid >> 3);
+    const int w = blockIdx.x;
     if (w >= nwork) return;
-    const int2 wd = work[w];
-    const BatchDom<typename Store<MODE>::R> &D = doms[wd.x];
+    const int4 it = work[w];  // (domain, tile, zs, ze | lead flag)
+    const BatchDom<typename Store<MODE>::R> &D = doms[it.x];
     const int par = D.parity ^ flip;
-    const uint32_t seed = seeds[2 * wd.x + flip];
-    if (wd.y == 0 && threadIdx.x == 0) seeds[2 * wd.x + (flip ^ 1)] = seed + 1u;
+    const uint32_t seed = seeds[2 * it.x + flip];
+    if ((it.w & BATCH_LEAD) && threadIdx.x == 0) seeds[2 * it.x + (flip ^ 1)] = seed + 1u;
     pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
-                                           D.mask2d, D.omega, D.P, wd.y, sr_key(seed));
+                                           D.mask2d, D.omega, D.P, it.y, it.z,
+                                           it.w & ~BATCH_LEAD, sr_key(seed));
 }
 
 template <int MODE>
@@ -1343,11 +1345,16 @@ size_t batch_dom_bytes(int precision)
 
 int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, int chunk,
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
-                   const int *tiles, int ntiles, int parity)
+                   const int *tiles, int ntiles, int parity, int split[3])
 {
     auto fill = [&](auto *D, auto tag) -> int {
         constexpr int M = decltype(tag)::value;
         fill_params<M>(D->P, g, a, chunk, tiles, ntiles);
+        if (split) {
+            split[0] = D->P.ntile;
+            split[1] = D->P.cq;
+            split[2] = D->P.cr;
+        }
         D->buf[0] = buf0;
         D->buf[1] = buf1;
         D->mask2d = mask2d;

@@ -243,7 +243,7 @@ int fslbm_fold_mask(int lx, int ly, int lz, int d2, const uint8_t *padded,
 /* Batched domains (no counterpart in the reference, which runs one     */
 /* hard-coded pipe): many independent pipes / periodic boxes advanced   */
 /* together, one launch per kernel class and step over a packed table  */
-/* of (domain, tile x z-chunk) work items, one workgroup per item.      */
+/* of (domain, tile, z range) work items, one workgroup per item.       */
 /* Every work item runs the same                                        */
 /* device code as the single-domain tiled kernel, so a domain's result  */
 /* is bitwise the same alone or in any mix.  Domains are admitted       */
@@ -282,8 +282,10 @@ int fslbm_batch_query(fslbm_batch_t b, size_t *used, size_t *budget,
  * graph launch). */
 int fslbm_batch_use_graph(fslbm_batch_t b, int enable);
 
-/* Schedule: z planes per work item (0: per-domain default), or pick the
- * fastest by timing trial launches that do not advance any domain. */
+/* Schedule: the longest work item in z planes (0: default, 128).  Tile
+ * columns are cut into items that shorten as the table fills (the last
+ * ones are at most 8 planes) and run longest first.  autotune picks the
+ * fastest cap by timing trial launches that do not advance any domain. */
 int fslbm_batch_set_chunk(fslbm_batch_t b, int chunk);
 int fslbm_batch_autotune(fslbm_batch_t b, int trial_steps, int *best_chunk,
                          double *best_ms);
