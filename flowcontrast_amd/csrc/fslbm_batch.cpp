@@ -125,9 +125,9 @@ struct Column {
 // Each cut takes L = (remaining planes of the class) / (2 * CUs) planes,
 // clamped to [lmin, lmax], from the column with the most planes left, so
 // item lengths shrink as the table fills and the last items are short.
-// The table is ordered longest first; workgroup b takes item b and the
-// dispatcher deals workgroups to XCD b mod 8, so each XCD gets an even
-// share and the step's tail is one short item.  One item per domain
+// The items are ordered longest first and dealt to the XCDs in groups (see
+// below), so each XCD gets an even share and the step's tail is one short
+// item.  One item per domain
 // carries BATCH_LEAD.  The cut changes only the schedule: every cell is
 // computed the same way whatever item holds it.
 std::vector<int4> guided_split(std::vector<Column> &cols, int lmax, int cus, size_t ndom)
@@ -159,17 +159,28 @@ std::vector<int4> guided_split(std::vector<Column> &cols, int lmax, int cus, siz
     }
     std::stable_sort(items.begin(), items.end(),
                      [](const auto &x, const auto &y) { return x.first > y.first; });
-    std::vector<int4> out;
-    out.reserve(items.size());
+    // Deal groups of G consecutive items (neighbouring tiles of one z
+    // range, which share halo planes in L2) to the XCDs round robin:
+    // group g goes to XCD g mod 8.  Workgroup b runs on XCD b mod 8, so
+    // item k of the sorted list sits at 8 * (its index on its XCD) + XCD;
+    // unused positions hold empty items (domain -1).
+    constexpr int G = BATCH_GROUP;
+    const size_t n = items.size();
+    const size_t ngroups = (n + G - 1) / G;
+    const size_t rounds = (ngroups + 7) / 8;
+    std::vector<int4> out(n ? 8 * rounds * G : 0, make_int4(-1, 0, 0, 0));
     std::vector<char> led(ndom, 0);
-    for (const auto &e : items) {
-        int4 it = e.second;
+    for (size_t k = 0; k < n; k++) {
+        int4 it = items[k].second;
         if (!led[it.x]) {
             led[it.x] = 1;
             it.w |= BATCH_LEAD;
         }
-        out.push_back(it);
+        const size_t g = k / G, xcd = g % 8, seq = (g / 8) * G + k % G;
+        out[8 * seq + xcd] = it;
     }
+    // trailing empty positions need no workgroup
+    while (!out.empty() && out.back().x < 0) out.pop_back();
     return out;
 }
 

@@ -91,6 +91,8 @@ int default_chunk(const Geo &g, int precision, int ntiles);
 // Flag on a batch work item's ze: the domain's lead item, whose workgroup
 // advances the domain's step counter.
 constexpr int BATCH_LEAD = 0x40000000;
+// Batch work items are dealt to the XCDs in groups of this many.
+constexpr int BATCH_GROUP = 16;
 
 // Batched step over many domains (32-bit arithmetic modes).  `doms` is a
 // device array of per-domain descriptors (batch_dom_bytes each, filled by

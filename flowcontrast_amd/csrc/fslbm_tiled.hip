@@ -1067,9 +1067,10 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
 // ---------------------------------------------------------------------
 // Batched step: many independent domains advanced by one launch over a
 // packed table of work items (domain, tile, z range), one workgroup per
-// item.  The host orders the table longest item first; workgroup b takes
-// item b, and the dispatcher deals workgroups to the 8 XCDs round robin,
-// so every XCD gets an even share and runs its items longest first.
+// item.  The host orders the items longest first and places them so that
+// groups of neighbouring tiles land on one XCD (workgroup b takes table
+// entry b and runs on XCD b mod 8) while every XCD gets an even share,
+// run longest first.
 // Every item runs pipe_item, the same code as the single-domain kernel,
 // so a domain's result does not depend on what else is in the batch.
 // ---------------------------------------------------------------------
@@ -1098,6 +1099,7 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const int w = blockIdx.x;
     if (w >= nwork) return;
     const int4 it = work[w];  // (domain, tile, zs, ze | lead flag)
+    if (it.x < 0) return;     // an empty position of the table
     const BatchDom<typename Store<MODE>::R> &D = doms[it.x];
     const int par = D.parity ^ flip;
     const uint32_t seed = seeds[2 * it.x + flip];
