@@ -137,7 +137,7 @@ def bench_batch(count: int, steps: int, sequential: bool):
             slots.append(b.add(prm, st, mask))
         upload = time.perf_counter() - t
         b.use_graph(True)
-        chunk, _ = b.autotune(2)
+        chunk, _ = b.autotune(10)
         b.sync()
         t0 = time.perf_counter()
         b.step(steps)
@@ -451,7 +451,7 @@ def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
                 series.append(0.5 * r[5] / cells)
             ke[prec] = np.array(series)
             if prec != "f64":
-                s.autotune(3)
+                s.autotune(10)
                 ms = s.time_steps(timed_steps) / timed_steps
                 B = algorithmic_bytes_per_cell(prec, False)
                 out[f"{prec}_mlups"] = round(cells / ms / 1e3, 1)
@@ -552,7 +552,7 @@ def main() -> int:
     if args.chunk > 0:
         s.set_tile_chunk(args.chunk)
     elif args.chunk < 0:  # autotune the schedule (state is not advanced)
-        s.autotune(5)
+        s.autotune(20)
     if args.precision == "f64":
         s.set_state(*st)
     else:
