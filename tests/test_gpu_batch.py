@@ -143,3 +143,28 @@ def test_batch_d1_1_equals_alone(precision):
         for slot, r in zip(slots, ref):
             assert all(np.isfinite(e).all() for e in r)
             assert all(np.array_equal(g, e) for g, e in zip(b.get_state(slot), r))
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16s"])
+def test_tiny_work_items_change_only_the_schedule(precision):
+    """Work items of 1 to 5 z planes (the solo kernel's chunk, the batch's
+    longest-item cap) give bitwise the state of the default schedule: a
+    cell's arithmetic does not depend on which item holds it."""
+    cases = [CASES[1], CASES[3]]  # order-4 arms in a pipe; a periodic box
+    refs = [alone(prm, mask, st, 6, precision) for prm, mask, st in cases]
+    for (prm, mask, st), ref in zip(cases, refs):
+        for ch in (1, 2, 3, 5):
+            with Solver(np.shape(st[0]), prm, mask=mask, precision=precision,
+                        kernel=KERNEL_TILED) as s:
+                s.set_tile_chunk(ch)
+                s.set_state(*st)
+                s.step(6)
+                got = s.get_state()
+            assert all(np.array_equal(g, e) for g, e in zip(got, ref)), ch
+    for cap in (1, 3):
+        with Batch(precision, 2, 8 << 30) as b:
+            slots = [b.add(prm, st, mask) for prm, mask, st in cases]
+            b.set_chunk(cap)
+            b.step(6)
+            for sl, ref in zip(slots, refs):
+                assert all(np.array_equal(g, e) for g, e in zip(b.get_state(sl), ref)), cap
