@@ -1260,7 +1260,9 @@ std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int c
     const long long T = ntiles > 0 ? ntiles
                                    : (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
     const int cus = device_cus();
-    const int minlen = std::min(16, g.nz);
+    // chunks of at least 4 planes: a domain too small to fill the GPU
+    // runs fastest in short chunks, whose pipelines run side by side
+    const int minlen = std::min(4, g.nz);
     std::vector<std::pair<double, int>> c;  // (cost, chunk length)
     for (int nch = 1; nch <= g.nz / minlen; nch++) {
         const int len = (g.nz + nch - 1) / nch;
