@@ -9,7 +9,7 @@ ABI in include/fslbm.h.
 """
 from ._lib import (BF16S, F16S, F32, F64, KERNEL_AUTO, KERNEL_PLAIN,  # noqa: F401
                    KERNEL_TILED, FslbmError, lib)
-from .solver import RunParams, Solver  # noqa: F401
+from .solver import RunParams, Solver, tile_plan  # noqa: F401
 from . import inputs  # noqa: F401
 from .batch import Batch, BudgetFull  # noqa: F401
 from .serve import Completion, Server  # noqa: F401

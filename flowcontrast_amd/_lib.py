@@ -21,7 +21,7 @@ EXPORTS = [
     "fslbm_abi_version", "fslbm_error_string", "fslbm_create", "fslbm_destroy",
     "fslbm_last_error", "fslbm_set_params", "fslbm_get_params",
     "fslbm_set_kernel", "fslbm_get_kernel", "fslbm_set_tile_chunk",
-    "fslbm_get_tile_chunk", "fslbm_step_cells", "fslbm_autotune", "fslbm_set_state",
+    "fslbm_get_tile_chunk", "fslbm_step_cells", "fslbm_tile_plan", "fslbm_autotune", "fslbm_set_state",
     "fslbm_get_state", "fslbm_set_state_f32", "fslbm_get_state_f32",
     "fslbm_step", "fslbm_set_gauge", "fslbm_monitor", "fslbm_reduce",
     "fslbm_probe", "fslbm_get_plane", "fslbm_stats_config",
@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
         "fslbm_set_tile_chunk": (i, [vp, i]),
         "fslbm_get_tile_chunk": (i, [vp]),
         "fslbm_step_cells": (C.c_longlong, [vp]),
+        "fslbm_tile_plan": (i, [i, i, i, i, C.c_void_p, ip, ip, C.POINTER(C.c_longlong), ip]),
         "fslbm_autotune": (i, [vp, i, ip, C.POINTER(d)]),
         "fslbm_set_state": (i, [vp, _dp, _dp, _dp, _dp]),
         "fslbm_get_state": (i, [vp, _dp, _dp, _dp, _dp]),

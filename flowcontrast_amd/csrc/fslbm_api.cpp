@@ -466,6 +466,27 @@ int fslbm_get_tile_chunk(fslbm_t c)
                         : default_chunk(c->g, c->precision, c->ntiles);
 }
 
+int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d,
+                    int *tile_rows, int *ntiles, long long *cells, int *chunk)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || precision < FSLBM_F64 || precision > FSLBM_BF16S)
+        return FSLBM_EINVAL;
+    Geo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.N = (long long)nx * ny * nz;
+    g.periodic = mask2d == nullptr;
+    const int rows = tiled_rows(precision);
+    const std::vector<int> tl = active_tiles(g, precision, mask2d);
+    const int all = ((nx + 63) / 64) * ((ny + rows - 1) / rows);
+    if (tile_rows) *tile_rows = rows;
+    if (ntiles) *ntiles = tl.empty() ? all : (int)tl.size();
+    if (cells) *cells = tl.empty() ? g.N : tiles_cells(g, precision, tl);
+    if (chunk) *chunk = default_chunk(g, precision, (int)tl.size());
+    return FSLBM_OK;
+}
+
 long long fslbm_step_cells(fslbm_t c)
 {
     if (!c) return FSLBM_EINVAL;

@@ -87,6 +87,8 @@ long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles
 // time over `ntiles` tiles (0: every tile); default_chunk is the first.
 std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count);
 int default_chunk(const Geo &g, int precision, int ntiles);
+// Rows of the tiled kernel's tiles (64 columns wide) in a precision mode.
+int tiled_rows(int precision);
 
 // Flag on a batch work item's ze: the domain's lead item, whose workgroup
 // advances the domain's step counter.

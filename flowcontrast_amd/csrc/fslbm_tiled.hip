@@ -1283,6 +1283,8 @@ std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int c
     return out;
 }
 
+int tiled_rows(int precision) { return tile_rows(precision); }
+
 int default_chunk(const Geo &g, int precision, int ntiles)
 {
     return chunk_candidates(g, precision, ntiles, 1)[0];

@@ -213,3 +213,21 @@ class Solver:
         self._chk(lib().fslbm_moments_read(self._h, out.ctypes.data_as(C.c_void_p),
                                            C.byref(s)))
         return out.reshape(11, ny, nx), s.value
+
+
+def tile_plan(shape, precision: str = "f32", mask=None) -> dict:
+    """The tiled kernel's schedule for a domain of shape (nz, ny, nx),
+    computed on the host: tile rows, tiles holding fluid, cells one step
+    moves and the default chunk.  `mask` is the z-invariant mask (one
+    (ny, nx) plane or the full volume), None for a periodic box."""
+    nz, ny, nx = shape
+    mptr = None
+    if mask is not None:
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8).reshape(-1, ny, nx)[0])
+        mptr = m.ctypes.data_as(C.c_void_p)
+    rows, ntiles, chunk = C.c_int(), C.c_int(), C.c_int()
+    cells = C.c_longlong()
+    check(lib().fslbm_tile_plan(nx, ny, nz, _lib.PRECISIONS[precision], mptr, C.byref(rows),
+                                C.byref(ntiles), C.byref(cells), C.byref(chunk)))
+    return {"tile_rows": rows.value, "tiles": ntiles.value, "cells": cells.value,
+            "chunk": chunk.value}

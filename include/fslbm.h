@@ -110,6 +110,17 @@ int fslbm_get_tile_chunk(fslbm_t ctx);
  * negative error code on failure. */
 long long fslbm_step_cells(fslbm_t ctx);
 
+/* Host-side plan of the tiled kernel's schedule for a domain (no reference
+ * counterpart; no device needed, for capacity planning and tests).
+ * mask2d: one z plane of nx*ny codes (0/125), or NULL for a periodic box.
+ * Outputs, any of which may be NULL: the rows of the mode's tiles (64
+ * columns wide), the number of tiles holding fluid (work items run on
+ * these only), the cells one step moves (as fslbm_step_cells) and the
+ * default chunk (z planes per work item).  FP64 runs the two-barrier
+ * kernel on 8-row tiles. */
+int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d,
+                    int *tile_rows, int *ntiles, long long *cells, int *chunk);
+
 /* Autotuner of the tiled kernel's schedule: times trial_steps launches for
  * each candidate chunk (HIP events on the context's stream), keeps the
  * fastest and reports it (*best_chunk, *best_ms per step; either may be
