@@ -15,7 +15,7 @@ import math
 import numpy as np
 import pytest
 
-from flowcontrast_amd import KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver, inputs
+from flowcontrast_amd import KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver, inputs, tile_plan
 from flowcontrast_amd._lib import FslbmError
 
 pytestmark = pytest.mark.gpu
@@ -204,6 +204,9 @@ def test_wall_only_tiles_are_skipped_exactly(orc, precision, tol):
         s.set_gauge(5e-4)
         s.step(7)
         got = s.get_state()
+        plan = tile_plan(mask.shape, precision, mask)
+        assert s.step_cells == plan["cells"] < mask.size
+        assert s.tile_chunk == plan["chunk"]
     wall = mask != 0
     for f in range(4):
         assert not np.any(np.asarray(got[f])[wall]), f
