@@ -552,7 +552,7 @@ def main() -> int:
     if args.chunk > 0:
         s.set_tile_chunk(args.chunk)
     elif args.chunk < 0:  # autotune the schedule (state is not advanced)
-        s.autotune(20)
+        s.autotune(50)
     if args.precision == "f64":
         s.set_state(*st)
     else:

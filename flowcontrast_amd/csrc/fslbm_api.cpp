@@ -439,6 +439,12 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
             cands.push_back(f);
     int best = 0;
     double bms = 1e300;
+    // untimed warm-up, so that the first candidate is not timed on an idle GPU
+    {
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, cands[0], c->stream};
+        for (int t = 0; t < 2 * trial_steps; t++)
+            HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
+    }
     for (int cand : cands) {
         TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, std::min(cand, c->g.nz), c->stream};
         // warm-up launch, then timed launches

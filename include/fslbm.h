@@ -121,8 +121,10 @@ long long fslbm_step_cells(fslbm_t ctx);
 int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d,
                     int *tile_rows, int *ntiles, long long *cells, int *chunk);
 
-/* Autotuner of the tiled kernel's schedule: times trial_steps launches for
- * each candidate chunk (HIP events on the context's stream), keeps the
+/* Autotuner of the tiled kernel's schedule: after 2*trial_steps untimed
+ * launches, times trial_steps launches for each candidate chunk (the
+ * step-time model's best picks and a few fixed lengths; HIP events on the
+ * context's stream), keeps the
  * fastest and reports it (*best_chunk, *best_ms per step; either may be
  * NULL).  The solver state is not advanced: trial launches read the
  * current state and write the spare buffer.  No-op for the plain kernel. */
