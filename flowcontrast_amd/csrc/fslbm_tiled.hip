@@ -42,6 +42,7 @@
 //   wave 10    halo columns x0-1 and x0+64 (20 lanes): z pass, x messages
 
 #include <algorithm>
+#include <atomic>
 #include <utility>
 
 #include "fslbm_internal.h"
@@ -1216,17 +1217,18 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
 
 int device_cus()
 {
-    static int cached[64] = {0};
+    // per device; contexts on several host threads may ask at once
+    static std::atomic<int> cached[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cached[dev] <= 0) {
-        int v = 0;
+    int v = cached[dev].load(std::memory_order_relaxed);
+    if (v <= 0) {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             v <= 0)
-            v = 256;
-        cached[dev] = v;
+            v = 256;  // MI355X
+        cached[dev].store(v, std::memory_order_relaxed);
     }
-    return cached[dev];
+    return v;
 }
 
 namespace {
