@@ -3,9 +3,11 @@
 // Same update as the plain kernel (FSLBM.cu:130-303 semantics), organised
 // for HBM bandwidth:
 //
-// * One workgroup owns a 64 x 8 (x, y) tile and marches over a chunk of
-//   z planes.  Every state value is read from HBM once per step (halo
-//   re-reads hit L2) and written once.
+// * One workgroup owns a 64 x TY (x, y) tile and marches over a chunk of
+//   z planes: TY = 8 in the two-barrier FP64 kernel (k_step_tiled), 12
+//   (FP32) or 8 (16-bit storage) in the pipelined kernel (k_step_pipe,
+//   further below).  Every state value is written once per step and read
+//   from HBM about once (halo and chunk-end re-reads mostly hit L2).
 // * The 27-point D3Q27 predictor is evaluated as three separable 3-tap
 //   passes.  D3Q27 weights are products w(cx) w(cy) w(cz) with
 //   w(0) = 2/3, w(+-1) = 1/6, and the equilibrium
@@ -13,7 +15,8 @@
 //   is a quadratic polynomial in e, so sum_e W_e g_e(x+e) (1, e) becomes:
 //     z pass  (registers, own column): 12 coefficients of a polynomial
 //             in (cx, cy) from the planes z-1, z, z+1;
-//     x pass  (LDS): each column publishes that polynomial evaluated at
+//     x pass  (LDS in k_step_tiled, DPP lane shifts in k_step_pipe):
+//             each column publishes that polynomial evaluated at
 //             cx = +1 and cx = -1 (10 values), neighbours gather;
 //     y pass  (LDS): each row publishes the cy = +1 / -1 values of the
 //             three remaining polynomials (6 values).
@@ -31,12 +34,12 @@
 //   run of work items (z-chunk major), so tiles that share halos share
 //   an L2.
 // * Work items run over the tiles that hold a fluid cell only: a tile of
-//   wall cells would read zeros and store nothing (walls are never
-//   written, and both state buffers hold zeros there).  The z planes split
+//   wall cells would read zeros and store zeros over the zeros both state
+//   buffers hold at walls from the upload on.  The z planes split
 //   into chunks whose lengths differ by at most one, and the chunk count
 //   is chosen so that the items fill whole rounds of workgroups.
 //
-// Roles of the 704 threads (11 waves):
+// Roles of the 704 threads (11 waves) of k_step_tiled:
 //   waves 0-7  interior columns (64 x 8): everything
 //   waves 8-9  halo rows y0-1 and y0+8: z pass, x pass, y messages
 //   wave 10    halo columns x0-1 and x0+64 (20 lanes): z pass, x messages
