@@ -460,6 +460,10 @@ def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
     eps_ref = -np.diff(ref) / every
     out["ke0"] = float(ref[0])
     out["ke_final_over_ke0_f64"] = float(ref[-1] / ref[0])
+    # the breakdown to turbulence: where the dissipation rate peaks
+    k_ref = int(np.argmax(eps_ref))
+    out["f64_peak_dissipation_step"] = (k_ref + 1) * every
+    out["f64_peak_dissipation"] = float(f"{float(eps_ref[k_ref]):.6e}")
     for prec in precisions:
         if prec == "f64":
             continue
@@ -467,6 +471,10 @@ def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
         out[f"{prec}_ke_max_rel_err"] = float(f"{float(np.max(np.abs(ke[prec] - ref) / ref)):.3e}")
         out[f"{prec}_dissipation_max_rel_err"] = float(
             f"{float(np.max(np.abs(eps - eps_ref)) / np.max(np.abs(eps_ref))):.3e}")
+        k = int(np.argmax(eps))
+        out[f"{prec}_peak_dissipation_step"] = (k + 1) * every
+        out[f"{prec}_peak_dissipation_rel_err"] = float(
+            f"{abs(float(eps[k]) - float(eps_ref[k_ref])) / float(eps_ref[k_ref]):.3e}")
     return out
 
 
@@ -498,6 +506,7 @@ def main() -> int:
     ap.add_argument("--spot-checks", type=int, default=2,
                     help="serving: requests of each precision checked against FP64")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--tgv-steps", type=int, default=5000)
     ap.add_argument("--stats-steps", type=int, default=50000)
     ap.add_argument("--stats-window", type=int, default=20000)
     ap.add_argument("--stats-every", type=int, default=100)
@@ -523,7 +532,8 @@ def main() -> int:
                                      not args.no_sequential)), flush=True)
         return 0
     if args.workload == "tgv":
-        print(json.dumps(bench_tgv(256, 5000, 250, 100, ["f64", "f32", "f16s"])), flush=True)
+        print(json.dumps(bench_tgv(256, args.tgv_steps, 250, 100, ["f64", "f32", "f16s"])),
+              flush=True)
         return 0
     if args.workload == "stats":
         print(json.dumps(bench_stats(args.stats_steps, args.stats_window, args.stats_every,
