@@ -520,10 +520,10 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} "
+        print(f"bench.py: the FSLBM step runs on one GPU; --gpus {args.gpus} "
               f"(WORLD_SIZE={world}) is not supported. Independent pipes or "
               f"requests shard one per GPU; a single domain would need a "
-              f"z-slab halo exchange that is not part of this build.",
+              f"z-slab halo exchange, which is not implemented.",
               file=sys.stderr)
         return 2
 
