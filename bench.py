@@ -5,7 +5,9 @@ circular wall of radius 120 centred in the cross-section, z periodic with
 the pressure jump pin, FP32, Poiseuille + 5% seeded noise initial state
 (synthetic; bulk speed 0.05 lattice units, nu = 0.05*2R/5300 for bulk
 Re ~ 5300, Blasius-estimated pin).
-One step = one FSLBM time step over all 67,108,864 lattice cells.
+One step = one FSLBM time step over all 67,108,864 lattice cells (the
+kernel reads and writes the 61,341,696 cells of the tiles that hold
+fluid; tiles of wall cells hold zeros and are skipped).
 
 Printed: one JSON line with MLUPS (all lattice cells, the paper's
 convention), the HBM roofline of the step kernel and the CPU baseline
@@ -119,10 +121,11 @@ def field_errors(got, ref):
 
 
 def bench_batch(count: int, steps: int, sequential: bool):
-    """Independent uneven pipes in one batch (persistent launch per step,
-    autotuned chunk, hipGraph replay) against the same pipes solved one
-    after another by the single-domain solver.  Timed: the steps only,
-    states resident on the device on both sides."""
+    """Independent uneven pipes in one batch (one launch per step over a
+    table of (pipe, tile, z range) items, autotuned longest-item cap,
+    hipGraph replay) against the same pipes solved one after another by
+    the single-domain solver.  Timed: the steps only, states resident on
+    the device on both sides."""
     from flowcontrast_amd import KERNEL_AUTO, KERNEL_PLAIN, Batch, Solver
     from flowcontrast_amd import workloads as W
     cases = W.uneven_pipes(count, 4, steps)
