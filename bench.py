@@ -187,7 +187,7 @@ def bench_batch(count: int, steps: int, sequential: bool):
 
 
 def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
-                sequential: bool):
+                sequential: bool, policy: str = "share", fill: int = 0):
     """Poisson stream of pipe / Taylor-Green solve requests through the
     serving loop (continuous batching under an HBM budget).  Latency is
     time to solution from each request's scheduled arrival to its
@@ -210,6 +210,7 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
         lat, status, kept = {}, {}, {}
         peak_used = peak_queue = 0
         with Server(d1=2, hbm_budget=budget, quantum=quantum) as srv:
+            srv.set_policy(policy, fill)
             t_start = srv.clock()
             owner, i = {}, 0
             while len(lat) < count:
@@ -242,7 +243,8 @@ def bench_serve(count: int, rate: float, quantum: int, budget: int, spot: int,
     out = {"workload": f"{count} requests, Poisson {rate}/s, 80% pipes (R U[16,64], "
                        f"L U[4,8] D, Re U[200,6000]) / 20% Taylor-Green (64^3-192^3, "
                        f"Re U[100,1600]), 2000-20000 steps, FP32 or FP16 storage, seed 5",
-           "requests": count, "rate_per_s": rate, "quantum": quantum,
+           "requests": count, "rate_per_s": rate, "quantum": quantum, "policy": policy,
+           "fill_cells": fill,
            "hbm_budget": budget, "cell_updates": work,
            "p50_s": round(float(np.percentile(l, 50)), 4),
            "p99_s": round(float(np.percentile(l, 99)), 4),
@@ -505,6 +507,11 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-quantum", type=int, default=200)
+    ap.add_argument("--serve-fill", type=int, default=0,
+                    help="shortest policy: cells stepped per pump at least (0: library default)")
+    ap.add_argument("--serve-policy", default="share", choices=["share", "shortest"],
+                    help="which resident requests step in a pump: least work left "
+                         "first up to a GPU's fill, or all of them")
     ap.add_argument("--serve-budget", type=int, default=32 << 30)
     ap.add_argument("--spot-checks", type=int, default=2,
                     help="serving: requests of each precision checked against FP64")
@@ -545,7 +552,8 @@ def main() -> int:
     if args.workload == "serve":
         print(json.dumps(bench_serve(args.serve_requests, args.serve_rate, args.serve_quantum,
                                      args.serve_budget, args.spot_checks,
-                                     not args.no_sequential)), flush=True)
+                                     not args.no_sequential, args.serve_policy,
+                                     args.serve_fill)), flush=True)
         return 0
 
     from flowcontrast_amd import KERNEL_AUTO, KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver
@@ -636,7 +644,9 @@ def main() -> int:
                         ("serving", lambda: bench_serve(args.serve_requests, args.serve_rate,
                                                         args.serve_quantum, args.serve_budget,
                                                         args.spot_checks,
-                                                        not args.no_sequential))):
+                                                        not args.no_sequential,
+                                                        args.serve_policy,
+                                                        args.serve_fill))):
             try:
                 out[key] = fn()
             except Exception as e:  # reported in the line, never hidden

@@ -35,12 +35,14 @@ EXPORTS = [
     "fslbm_batch_step", "fslbm_batch_get_state", "fslbm_batch_query",
     "fslbm_batch_use_graph", "fslbm_batch_set_chunk", "fslbm_batch_autotune",
     "fslbm_batch_time_steps", "fslbm_batch_sync",
-    "fslbm_server_create", "fslbm_server_destroy", "fslbm_server_last_error",
+    "fslbm_server_create", "fslbm_server_set_policy", "fslbm_server_destroy",
+    "fslbm_server_last_error",
     "fslbm_server_clock", "fslbm_server_submit", "fslbm_server_pump",
     "fslbm_server_next_done", "fslbm_server_fetch", "fslbm_server_drop",
     "fslbm_server_query",
 ]
 EBUSY = -6
+SERVE_SHARE, SERVE_SHORTEST = 0, 1  # serving policies
 
 
 class Params(C.Structure):
@@ -157,6 +159,7 @@ def lib() -> C.CDLL:
         "fslbm_batch_time_steps": (i, [vp, i, C.POINTER(d)]),
         "fslbm_batch_sync": (i, [vp]),
         "fslbm_server_create": (i, [i, C.c_size_t, i, C.POINTER(vp)]),
+        "fslbm_server_set_policy": (i, [vp, i, C.c_longlong]),
         "fslbm_server_destroy": (None, [vp]),
         "fslbm_server_last_error": (C.c_char_p, [vp]),
         "fslbm_server_clock": (d, [vp]),

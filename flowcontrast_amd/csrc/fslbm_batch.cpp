@@ -723,6 +723,22 @@ int batch_hold(fslbm_batch *b, int slot)
     return FSLBM_OK;
 }
 
+// Step a held domain again from the buffer it was held in.  Its step
+// counter moved only while it stepped, so its results are those of an
+// uninterrupted run.
+int batch_resume(fslbm_batch *b, int slot)
+{
+    if (!b) return FSLBM_EINVAL;
+    if (slot < 0 || slot >= (int)b->doms.size() || !b->doms[slot].active)
+        return bfail(b, FSLBM_EINVAL, "no domain in slot %d", slot);
+    Domain &d = b->doms[slot];
+    if (!d.held) return FSLBM_OK;
+    d.parity = d.held_cur ^ b->flip;  // current buffer = parity ^ flip
+    d.held = false;
+    b->dirty = true;
+    return FSLBM_OK;
+}
+
 // Enqueue the copy of a held domain's state to out.raw on stream s (the
 // caller synchronises s before reading out.raw or removing the domain).
 int batch_copy_out(fslbm_batch *b, int slot, HostResult &out, hipStream_t s)

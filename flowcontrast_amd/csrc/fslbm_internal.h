@@ -131,6 +131,7 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync);
 // Stop stepping a domain, keeping it resident; then copy its state out on
 // a stream of the caller's (asynchronous).
 int batch_hold(fslbm_batch *b, int slot);
+int batch_resume(fslbm_batch *b, int slot);
 int batch_copy_out(fslbm_batch *b, int slot, HostResult &out, hipStream_t s);
 // Admit a domain whose mask (NULL: periodic box) is one z plane of nx*ny
 // codes; src holds the four initial fields.  Returns the slot or an error.

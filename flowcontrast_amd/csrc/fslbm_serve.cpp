@@ -17,6 +17,16 @@
 // its stepping requests, capped by the quantum, so no request runs past
 // its budget.  Device memory of a request counts against the budget from
 // its upload to its hand-back.
+//
+// Which resident requests step in a pump is the policy's choice:
+//   FSLBM_SERVE_SHARE (default): all of them (every request advances at
+//     the rate of the whole mix; the highest throughput);
+//   FSLBM_SERVE_SHORTEST: the requests with the least work left
+//     (cells x steps left) are taken first until their cells fill the GPU;
+//     the others wait resident, held in their batch, and resume from the
+//     buffer they stopped in.  A request that waited WAIT_MAX pumps in a row
+//     steps regardless, so none starves.  Pausing changes no arithmetic:
+//     results stay bitwise those of a solo run.
 
 #include <hip/hip_runtime.h>
 
@@ -38,6 +48,7 @@ using namespace fslbm;
 namespace {
 
 constexpr int NMODES = 3;  // FSLBM_F32, FSLBM_F16S, FSLBM_BF16S
+constexpr int WAIT_MAX = 32;  // pumps a running request may wait at most
 
 int mode_index(int precision)
 {
@@ -53,6 +64,8 @@ enum class RState { Queued, Ready, Running, Held, Done, Failed };
 
 struct Request {
     fslbm_request r{};
+    bool paused = false;  // resident and running, held in its batch this pump
+    int waited = 0;       // consecutive pumps paused
     std::vector<uint8_t> mask2d;  // one z plane; empty for a periodic box
     std::vector<double> st[4];    // initial state until admission
     int mode = 0;
@@ -73,6 +86,8 @@ thread_local std::string g_server_err;
 struct fslbm_server {
     int d1 = 2;
     int quantum = 100;
+    int policy = FSLBM_SERVE_SHARE;
+    long long fill_cells = 0;  // 0: 32768 cells per compute unit
     size_t budget = 0, used = 0;
     fslbm_batch_t batch[NMODES] = {nullptr, nullptr, nullptr};
     std::deque<long long> queue;        // arrival order
@@ -171,9 +186,60 @@ int prepare_queued(fslbm_server *s)
     return failed;
 }
 
+long long cells_of(const Request &q) { return (long long)q.r.nx * q.r.ny * q.r.nz; }
+
+// Choose this pump's stepping requests (in admission order) and pause or
+// resume the running requests' batch domains to match.
+int select_stepping(fslbm_server *s, std::vector<long long> &stepping)
+{
+    stepping.clear();
+    std::vector<char> go(s->running.size(), 1);
+    if (s->policy == FSLBM_SERVE_SHORTEST && !s->running.empty()) {
+        std::vector<std::pair<double, size_t>> order;  // (work left, index)
+        for (size_t k = 0; k < s->running.size(); k++) {
+            const Request &q = s->reqs.at(s->running[k]);
+            order.push_back({(double)cells_of(q) * (double)q.left, k});
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const auto &a, const auto &b) { return a.first < b.first; });
+        const long long fill =
+            s->fill_cells > 0 ? s->fill_cells : (long long)device_cus() * 32768;
+        long long cells = 0;
+        for (const auto &e : order) {
+            const Request &q = s->reqs.at(s->running[e.second]);
+            const bool take = cells < fill || q.waited >= WAIT_MAX;
+            go[e.second] = take;
+            if (take) cells += cells_of(q);
+        }
+    }
+    for (size_t k = 0; k < s->running.size(); k++) {
+        const long long id = s->running[k];
+        Request &q = s->reqs.at(id);
+        int rc = FSLBM_OK;
+        if (go[k] && q.paused) rc = batch_resume(s->batch[q.mode], q.slot);
+        if (!go[k] && !q.paused) rc = batch_hold(s->batch[q.mode], q.slot);
+        if (rc) return sfail(s, rc, "pause/resume: %s", fslbm_batch_last_error(s->batch[q.mode]));
+        q.paused = !go[k];
+        q.waited = go[k] ? 0 : q.waited + 1;
+        if (go[k]) stepping.push_back(id);
+    }
+    return FSLBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int fslbm_server_set_policy(fslbm_server_t s, int policy, long long fill_cells)
+{
+    if (!s) return FSLBM_EINVAL;
+    if (policy != FSLBM_SERVE_SHARE && policy != FSLBM_SERVE_SHORTEST)
+        return sfail(s, FSLBM_EINVAL, "unknown policy %d", policy);
+    if (fill_cells < 0) return sfail(s, FSLBM_EINVAL, "fill_cells must be >= 0");
+    s->policy = policy;
+    s->fill_cells = fill_cells;
+    return FSLBM_OK;
+}
 
 int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *out)
 {
@@ -259,9 +325,13 @@ long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req, const 
 int fslbm_server_pump(fslbm_server_t s)
 {
     if (!s) return FSLBM_EINVAL;
-    // 1. launch the steps of the stepping requests (one stream per mode)
+    // 1. choose the stepping requests and launch their steps (one stream
+    // per mode)
+    std::vector<long long> stepping;
+    int rc0 = select_stepping(s, stepping);
+    if (rc0) return rc0;
     long long qs[NMODES] = {0, 0, 0};
-    for (long long id : s->running) {
+    for (long long id : stepping) {
         const Request &q = s->reqs.at(id);
         long long &m = qs[q.mode];
         m = m == 0 ? q.left : std::min(m, q.left);
@@ -328,6 +398,10 @@ int fslbm_server_pump(fslbm_server_t s)
     still.reserve(s->running.size() + uploaded.size());
     for (long long id : s->running) {
         Request &q = s->reqs.at(id);
+        if (q.paused) {  // did not step in this pump
+            still.push_back(id);
+            continue;
+        }
         q.left -= qs[q.mode];
         if (q.left > 0) {
             still.push_back(id);

@@ -60,6 +60,14 @@ class Server:
     def _chk(self, rc):
         return check_server(rc, self._h)
 
+    def set_policy(self, policy: str = "share", fill_cells: int = 0) -> None:
+        """Which resident requests step in a pump: "share" (all of them, the
+        default) or "shortest" (least work left first, until fill_cells
+        cells; 0 = the library's default).  Results do not depend on the
+        policy."""
+        code = {"share": _lib.SERVE_SHARE, "shortest": _lib.SERVE_SHORTEST}[policy]
+        check_server(lib().fslbm_server_set_policy(self._h, code, int(fill_cells)), self._h)
+
     def clock(self) -> float:
         return lib().fslbm_server_clock(self._h)
 

@@ -338,6 +338,17 @@ typedef struct fslbm_completion {
  * resident requests may hold together. */
 int fslbm_server_create(int d1, size_t hbm_budget, int quantum,
                         fslbm_server_t *out);
+
+/* Which resident requests step in a pump.  FSLBM_SERVE_SHARE (the
+ * default): all of them, for the highest throughput.
+ * FSLBM_SERVE_SHORTEST: those with the least work left (cells x steps
+ * left) first, until their cells reach fill_cells (0: 32768 per compute
+ * unit); the others wait resident (at most 32 pumps in a row), for lower
+ * time to solution at some throughput.  Results do not depend on the
+ * policy. */
+#define FSLBM_SERVE_SHARE 0
+#define FSLBM_SERVE_SHORTEST 1
+int fslbm_server_set_policy(fslbm_server_t s, int policy, long long fill_cells);
 void fslbm_server_destroy(fslbm_server_t s);
 const char *fslbm_server_last_error(fslbm_server_t s);
 /* Seconds since the server was created (the clock of the completions). */

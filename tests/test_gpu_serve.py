@@ -54,8 +54,13 @@ def drain(srv, limit=10000):
     return done
 
 
-def test_results_equal_solo_runs():
+@pytest.mark.parametrize("policy, fill", [("share", 0), ("shortest", 0), ("shortest", 1)])
+def test_results_equal_solo_runs(policy, fill):
+    """With fill 1 the shortest-work policy steps one request per pump (and
+    any that waited too long), so requests are paused and resumed many
+    times; results stay bitwise those of solo runs."""
     with Server(d1=2, hbm_budget=8 << 30, quantum=16) as srv:
+        srv.set_policy(policy, fill)
         ids = [srv.submit(prm, st, mask, prec, steps)
                for (prm, mask, st), prec, steps in REQS]
         done = drain(srv)
@@ -120,6 +125,10 @@ def test_invalid_requests():
             srv.submit(prm, st, mask, "f64", 10)
         with pytest.raises(FslbmError):
             srv.submit(prm, st, mask, "f32", 0)
+        with pytest.raises(FslbmError):  # unknown policy, negative fill
+            srv.set_policy("shortest", -1)
+        from flowcontrast_amd._lib import lib
+        assert lib().fslbm_server_set_policy(srv._h, 7, 0) < 0
         # too short for the tiled kernels: fails at admission, alone
         short = [a[:8] for a in st]
         bad = srv.submit(prm, short, mask[:8], "f32", 10)
