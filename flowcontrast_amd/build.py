@@ -89,7 +89,7 @@ def build_cli() -> Path:
     if not src.exists():
         return CLI
     tmp = CLI.with_name(CLI.name + ".tmp")
-    _run([HIPCC, *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(tmp),
+    _run([HIPCC, *HOST_FLAGS, "-pthread", "-D__HIP_PLATFORM_AMD__", str(src), "-o", str(tmp),
           f"-L{PKG}", "-lfslbm", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, CLI)
     return CLI

@@ -22,18 +22,25 @@
 //                   repeats a continuous one bitwise (FP64 and FP32 modes).
 // The geometry sizes, hard-coded in the reference (FSLBM.cu:512-514), are
 // options here.  param.txt is re-read every 500 steps (FSLBM.cu:669-677).
+// The statistics and plane files are read from the device at their step
+// and written by a background thread while the steps go on (the text is
+// the reference's %e format); IO.txt, the constant mask plane, is written
+// once.
 //
 // usage: fslbm_run [--lx 120] [--ly 120] [--lz 320] [--d2 10]
 //                  [--steps N] [--precision f32|f64|f16s|bf16s]
 //                  [--monitor 50] [--stats 500] [--backup 50000]
 //                  [--fresh] [--quiet]
 
+#include <charconv>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fslbm.h"
@@ -115,13 +122,49 @@ std::vector<uint8_t> read_mask(const Opts &o)
     return m;
 }
 
-void write_plane(const char *name, const std::vector<double> &v)
+// One value as printf's "%e\n" writes it (six decimals, exponent of at
+// least two digits: std::to_chars rounds the same exact value the same
+// way), without printf's per-call cost.
+void append_e(std::string &out, double v)
 {
-    FILE *f = std::fopen(name, "wt");
-    if (!f) die("cannot write output file");
-    for (double x : v) std::fprintf(f, "%e\n", (float)x);
+    char b[48];
+    const auto r = std::to_chars(b, b + sizeof b, v, std::chars_format::scientific, 6);
+    out.append(b, r.ptr);
+    out.push_back('\n');
+}
+
+void write_text(const std::string &name, const std::string &text)
+{
+    FILE *f = std::fopen(name.c_str(), "wt");
+    if (!f || std::fwrite(text.data(), 1, text.size(), f) != text.size())
+        die("cannot write output file");
     std::fclose(f);
 }
+
+// A plane as the reference writes it: float values, one "%e" per line.
+void write_plane(const char *name, const std::vector<double> &v)
+{
+    std::string t;
+    t.reserve(v.size() * 14);
+    for (double x : v) append_e(t, (float)x);
+    write_text(name, t);
+}
+
+// Writes one job at a time on a thread of its own; a new job waits for
+// the previous one.
+struct BackgroundWriter {
+    std::thread t;
+    void run(std::function<void()> job)
+    {
+        wait();
+        t = std::thread(std::move(job));
+    }
+    void wait()
+    {
+        if (t.joinable()) t.join();
+    }
+    ~BackgroundWriter() { wait(); }
+};
 
 void copy_file(const char *src, const char *dst)
 {
@@ -219,25 +262,40 @@ const char *const AVG_NAMES[11] = {"_p", "_ux", "_uy", "_uz", "_p2", "_ux2",
                                    "_uy2", "_uz2", "_uxy2", "_uxz2", "_uyz2"};
 
 // FSLBM.cu:752-764 (and the backup copy, :886-899).
+// The statistics sums of one sample step, read from the device.
+struct Averages {
+    int n = 0, cnt = 0, cnt2 = 0;
+    std::vector<double> acc;  // 11 arrays of n sums
+};
+
+Averages read_averages(fslbm_t ctx)
+{
+    Averages a;
+    a.n = fslbm_stats_points(ctx);
+    a.acc.resize(11 * (size_t)a.n);
+    check(fslbm_stats_read(ctx, a.acc.data(), &a.cnt, &a.cnt2), ctx, "stats read");
+    return a;
+}
+
+// sum/cnt of each statistic, one "%e" per line, and the counts
+// (FSLBM.cu:752-764).
+void write_average_files(const Averages &a, const char *suffix)
+{
+    std::string t;
+    for (int k = 0; k < 11; k++) {
+        const double c = k < 4 ? (double)a.cnt : (double)a.cnt2;
+        t.clear();
+        t.reserve((size_t)a.n * 14);
+        for (int i = 0; i < a.n; i++) append_e(t, a.acc[k * (size_t)a.n + i] / c);
+        write_text(std::string(AVG_NAMES[k]) + suffix + ".txt", t);
+    }
+    write_text(std::string("_av") + suffix + ".txt",
+               std::to_string(a.cnt) + " " + std::to_string(a.cnt2) + "\n");
+}
+
 void write_averages(fslbm_t ctx, const char *suffix)
 {
-    const int n = fslbm_stats_points(ctx);
-    std::vector<double> acc(11 * (size_t)n);
-    int cnt = 0, cnt2 = 0;
-    check(fslbm_stats_read(ctx, acc.data(), &cnt, &cnt2), ctx, "stats read");
-    for (int k = 0; k < 11; k++) {
-        const std::string name = std::string(AVG_NAMES[k]) + suffix + ".txt";
-        FILE *f = std::fopen(name.c_str(), "wt");
-        if (!f) die("cannot write averages");
-        const double c = k < 4 ? (double)cnt : (double)cnt2;
-        for (int i = 0; i < n; i++) std::fprintf(f, "%e\n", acc[k * (size_t)n + i] / c);
-        std::fclose(f);
-    }
-    const std::string av = std::string("_av") + suffix + ".txt";
-    FILE *f = std::fopen(av.c_str(), "wt");
-    if (!f) die("cannot write _av.txt");
-    std::fprintf(f, "%d %d\n", cnt, cnt2);
-    std::fclose(f);
+    write_average_files(read_averages(ctx), suffix);
 }
 
 // FSLBM.cu:429-468: averages are restored as value * count.
@@ -353,8 +411,10 @@ int main(int argc, char **argv)
     { FILE *q = std::fopen("qp.txt", "a"); if (q) std::fclose(q); }
 
     const int n_yz = ny * o.lz;
-    std::vector<double> yz(n_yz), xy(nxy);
+    std::vector<double> xy(nxy);
     double pending_gauge = 0.0;  // p2 set by the last monitoring step
+    BackgroundWriter writer;     // statistics and plane files
+    bool io_written = false;
     // FSLBM.cu:630-764: monitoring, and every `stats` steps the statistics
     // and plane files
     auto monitor_step = [&](long c) {
@@ -381,23 +441,32 @@ int main(int argc, char **argv)
             prm = read_params();
             check(fslbm_set_params(ctx, &prm), ctx, "set params");
             check(fslbm_stats_accumulate(ctx, prm.delta), ctx, "stats");
-            write_averages(ctx, "");
-            // FSLBM.cu:772-821: instantaneous planes and the mask plane
-            const char *yzn[4] = {"p.txt", "ux.txt", "uy.txt", "uz.txt"};
-            const char *xyn[4] = {"p1.txt", "ux1.txt", "uy1.txt", "uz1.txt"};
+            // the averages and, FSLBM.cu:772-821, the instantaneous planes and
+            // the mask plane: read now, written while the steps go on
+            Averages av = read_averages(ctx);
+            std::vector<std::vector<double>> planes(8);
             for (int f = 0; f < 4; f++) {
-                check(fslbm_get_plane(ctx, 0, o.lx / 2, o.d2, f, yz.data()), ctx, "plane");
-                write_plane(yzn[f], yz);
-                check(fslbm_get_plane(ctx, 2, o.lz / 2, o.d2, f, xy.data()), ctx, "plane");
-                write_plane(xyn[f], xy);
+                planes[f].resize(n_yz);
+                check(fslbm_get_plane(ctx, 0, o.lx / 2, o.d2, f, planes[f].data()), ctx, "plane");
+                planes[4 + f].resize(nxy);
+                check(fslbm_get_plane(ctx, 2, o.lz / 2, o.d2, f, planes[4 + f].data()), ctx,
+                      "plane");
             }
-            FILE *io = std::fopen("IO.txt", "wt");
-            if (io) {
+            std::string io;  // constant: written once
+            if (!io_written) {
                 for (int z = 0; z < o.lz; z++)
                     for (int y = 0; y < ny; y++)
-                        std::fprintf(io, "%d\n", padded[o.lx / 2 + (size_t)o.lx * (y + (size_t)ny * z)]);
-                std::fclose(io);
+                        io += std::to_string(padded[o.lx / 2 + (size_t)o.lx * (y + (size_t)ny * z)]) +
+                              "\n";
+                io_written = true;
             }
+            writer.run([av = std::move(av), planes = std::move(planes), io = std::move(io)] {
+                static const char *names[8] = {"p.txt",  "ux.txt",  "uy.txt",  "uz.txt",
+                                               "p1.txt", "ux1.txt", "uy1.txt", "uz1.txt"};
+                write_average_files(av, "");
+                for (int k = 0; k < 8; k++) write_plane(names[k], planes[k]);
+                if (!io.empty()) write_text("IO.txt", io);
+            });
         }
     };
     for (long c = c0 + 1; c <= c0 + o.steps; c++) {
@@ -422,6 +491,7 @@ int main(int argc, char **argv)
             write_checkpoint(ctx, nx, ny, nz, o.precision, c, pending_gauge);
         }
     }
+    writer.wait();
     check(fslbm_sync(ctx), ctx, "sync");
     fslbm_destroy(ctx);
     return 0;
