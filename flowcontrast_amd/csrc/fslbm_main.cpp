@@ -106,19 +106,25 @@ fslbm_params read_params()
 // value too many; here exactly lx*ly*lz values are required).
 std::vector<uint8_t> read_mask(const Opts &o)
 {
-    FILE *f = std::fopen("pipe.txt", "r");
+    FILE *f = std::fopen("pipe.txt", "rb");
     if (!f) die("cannot open pipe.txt");
+    std::string text;
+    char buf[1 << 16];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, k);
+    std::fclose(f);
     const size_t n = (size_t)o.lx * o.ly * o.lz;
     std::vector<uint8_t> m(n);
+    const char *p = text.data(), *end = p + text.size();
     for (size_t i = 0; i < n; i++) {
-        int v;
-        if (std::fscanf(f, "%d", &v) != 1) {
-            std::fclose(f);
-            die("pipe.txt: fewer values than lx*ly*lz");
-        }
+        while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+        if (p == end) die("pipe.txt: fewer values than lx*ly*lz");
+        int v = 0;
+        const auto r = std::from_chars(p, end, v);
+        if (r.ec != std::errc()) die("pipe.txt: not an integer");
+        p = r.ptr;
         m[i] = (uint8_t)v;
     }
-    std::fclose(f);
     return m;
 }
 
