@@ -1112,11 +1112,16 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const int4 it = work[w];  // (domain, tile, zs, ze | lead flag)
     if (it.x < 0) return;     // an empty position of the table
     const BatchDom<typename Store<MODE>::R> &D = doms[it.x];
+    // The parameters are copied out of the descriptor before the first
+    // store: read through the reference, every use in the z loop would be
+    // reloaded after each buffer store (which may alias the descriptor),
+    // and each reload's wait also waits for the LDS reads in flight.
+    const TiledParams<typename Store<MODE>::R> P = D.P;
     const int par = D.parity ^ flip;
     const uint32_t seed = seeds[2 * it.x + flip];
     if ((it.w & BATCH_LEAD) && threadIdx.x == 0) seeds[2 * it.x + (flip ^ 1)] = seed + 1u;
     pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
-                                           D.mask2d, D.omega, D.P, it.y, it.z,
+                                           D.mask2d, D.omega, P, it.y, it.z,
                                            it.w & ~BATCH_LEAD, sr_key(seed));
 }
 
