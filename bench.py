@@ -10,7 +10,8 @@ kernel reads and writes the 61,341,696 cells of the tiles that hold
 fluid; tiles of wall cells hold zeros and are skipped).
 
 Printed: one JSON line with MLUPS (all lattice cells, the paper's
-convention), the HBM roofline of the step kernel and the CPU baseline
+convention), the HBM roofline of the step kernel (against the 8 TB/s peak,
+and against the measured rate of a plain device copy) and the CPU baseline
 (the C oracle with OpenMP on a bounded sample of the same workload), with
 sub-objects for the small 32x32x64 FP64 pipe (GPU against the CPU oracle
 at steps 100 and 1000), the 256^3 Taylor-Green box (FP32 / FP16 storage
@@ -94,6 +95,41 @@ def copy_calibration() -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_per_dispatch": 1 << 30, "dispatches": 10}))
     return 0
+
+
+def copy_ceiling() -> float:
+    """Read+write rate of a plain device copy of 2^30 bytes (torch's copy
+    kernel, 20 copies timed with HIP events on torch's stream): the
+    bandwidth a streaming kernel reaches on this GPU, beside the 8 TB/s
+    peak."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / 20
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * (1 << 30) / sec / 1e9
+
+
+def copy_ceiling_child():
+    """copy_ceiling() in a child process (GB/s, or None if it failed)."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--copy-rate"],
+                           capture_output=True, text=True, timeout=300)
+        return float(json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"])
+    except Exception as e:  # reported, never fatal to the benchmark
+        print(f"bench: copy rate not measured ({type(e).__name__})", file=sys.stderr)
+        return None
 
 
 def rel_l2(a, b) -> float:
@@ -521,12 +557,21 @@ def main() -> int:
     ap.add_argument("--stats-window", type=int, default=20000)
     ap.add_argument("--stats-every", type=int, default=100)
     ap.add_argument("--stats-precisions", default="f64,f32,f16s")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the plain-copy rate beside the roofline (torch's kernels "
+                         "crash the host side of rocprofv3 --kernel-trace)")
+    ap.add_argument("--copy-rate", action="store_true",
+                    help="print only the plain-copy rate (run by the benchmark in a child "
+                         "process)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB x 10 (the "
                          "scale for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    if args.copy_rate:
+        print(json.dumps({"copy_gbs": copy_ceiling()}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -569,6 +614,10 @@ def main() -> int:
     n = nx * ny * nz
     workload = f"pipe{nx}x{ny}x{nz}_r{int(R)}"
 
+    # The copy rate runs in a child process before this one touches the
+    # GPU: torch carries its own HIP runtime, which finds no device in a
+    # process where libfslbm's runtime already holds it.
+    ceiling = None if args.no_ceiling else copy_ceiling_child()
     s = Solver(mask.shape, prm, mask=mask, precision=args.precision, kernel=kern)
     if args.chunk > 0:
         s.set_tile_chunk(args.chunk)
@@ -632,6 +681,9 @@ def main() -> int:
         "mflups_fluid": round(nf * nz * args.steps / wall / 1e6, 1),
     }
     s.close()
+    if ceiling:
+        out["roofline"]["copy_gbs"] = round(ceiling, 1)
+        out["roofline"]["frac_of_copy"] = round(achieved / ceiling, 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mask, st, pin, nu, args.cpu_steps,
                                            args.precision)

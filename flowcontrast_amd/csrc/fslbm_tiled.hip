@@ -234,11 +234,13 @@ __device__ __forceinline__ uint16_t bload<uint16_t>(__amdgpu_buffer_rsrc_t r, un
 {
     return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
 }
-// The new state is stored non-temporal (cache-policy bit NT): nothing in
+// FP32 state is stored non-temporal (cache-policy bit NT): nothing in
 // this launch reads it back, so its lines should not displace the input
 // planes that neighbouring tiles still read from L2 (halo rows and
-// columns).  Measured +4.6% on the FP32 step of a 2 GB pipe, +1% in FP16
-// storage; non-temporal loads lose that sharing and measured 30% slower.
+// columns).  Measured +4.6% on the FP32 step of a 2 GB pipe and +6% on the
+// 256^3 Taylor-Green box.  16-bit stores stay ordinary: with NT they wrote
+// 8% more bytes and the FP16-storage Taylor-Green step ran 10% slower.
+// Non-temporal loads lose the halo sharing and measured 30% slower.
 constexpr int STORE_NT = 2;
 __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
@@ -246,12 +248,11 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsign
 }
 __device__ __forceinline__ void bstore(_Float16 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff,
-                                          STORE_NT);
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
 }
 __device__ __forceinline__ void bstore(uint16_t v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, voff, soff, STORE_NT);
+    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, voff, soff, 0);
 }
 
 // Planes of look-ahead beyond the corrector reach, chosen so the queue
