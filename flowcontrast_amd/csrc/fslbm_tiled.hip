@@ -751,6 +751,9 @@ __device__ __forceinline__ void pipe_item(
         };
 #pragma unroll
         for (int j = 2; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j);
+        // the last plane any stage of this chunk reads: corrector arm of
+        // the last output plane ze-1
+        const int qlast = ze - 1 + HF;
         Msg<R> mq[3];
         mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, fluid);
         mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fluid);
@@ -779,7 +782,10 @@ __device__ __forceinline__ void pipe_item(
                 yp1 = fd[cb][cy_ + D1][cx_]; ym1 = fd[cb][cy_ - D1][cx_];
             }
             __builtin_amdgcn_sched_barrier(0);
-            qload(a + HF + LL, (k + 2 * HF + 2 + LL) % QL);
+            // Planes past qlast feed only results past the chunk: their
+            // loads re-read plane qlast (a cache hit) instead of fetching
+            // planes the next chunk reads again.
+            qload(a + HF + LL < qlast ? a + HF + LL : qlast, (k + 2 * HF + 2 + LL) % QL);
             {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
                 mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fluid);
@@ -925,7 +931,9 @@ __device__ __forceinline__ void pipe_item(
             {
                 const int s1 = (k + 2) % PD;  // plane a+1
                 mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
-                hload(a + 1 + PD, s1);
+                // messages past plane ze and arms past ze-1 feed only
+                // results past the chunk: re-read those planes instead
+                hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
             }
             if (doB) {
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
@@ -940,7 +948,7 @@ __device__ __forceinline__ void pipe_item(
                     fd[(a - 1) & 1][fy][HF + ix] = cc;
                 }
             }
-            aload(a + 2, k % AD);
+            aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
             keep(zpass<R>(mq[k % 3], mq[(k + 1) % 3], mq[(k + 2) % 3]), xv, cen);
             __syncthreads();
         };
@@ -1015,7 +1023,9 @@ __device__ __forceinline__ void pipe_item(
             {
                 const int s1 = (k + 2) % PD;
                 mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
-                hload(a + 1 + PD, s1);
+                // messages past plane ze and arms past ze-1 feed only
+                // results past the chunk: re-read those planes instead
+                hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
             }
             if (doB) {
                 const int sa = k % AD;
@@ -1029,7 +1039,7 @@ __device__ __forceinline__ void pipe_item(
                     fd[(a - 1) & 1][afy[j]][afx[j]] = cc;
                 }
             }
-            aload(a + 2, k % AD);
+            aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
             if (writer) {
                 // edge block of this column: the values its neighbour lane
                 // reads (left column -> XL, right column -> XR); two
