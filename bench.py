@@ -13,9 +13,10 @@ Printed: one JSON line with MLUPS (all lattice cells, the paper's
 convention), the HBM roofline of the step kernel (against the 8 TB/s peak,
 and against the measured rate of a plain device copy) and the CPU baseline
 (the C oracle with OpenMP on a bounded sample of the same workload), with
-sub-objects for the small 32x32x64 FP64 pipe (GPU against the CPU oracle
-at steps 100 and 1000), the 256^3 Taylor-Green box (FP32 / FP16 storage
-against FP64), the 64 uneven pipes in one batch, and the serving stream.
+sub-objects for the 16-bit storage modes on the same pipe, the small
+32x32x64 FP64 pipe (GPU against the CPU oracle at steps 100 and 1000), the
+256^3 Taylor-Green box (FP32 / FP16 / BF16 storage against FP64), the 64
+uneven pipes in one batch, and the serving stream.
 Other workloads: --workload pipe|tgv|batch|serve|stats.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f32]
@@ -468,6 +469,33 @@ def bench_small_pipe():
     return out
 
 
+def bench_storage_modes(mask, st, prm, steps: int, warmup: int, f32_mlups: float):
+    """The 16-bit storage modes (FP32 arithmetic) on the same pipe and
+    initial state as the FP32 line: step rate, roofline fraction over
+    their 17 bytes per cell, and the ratio to the FP32 rate."""
+    from flowcontrast_amd import KERNEL_AUTO, Solver
+    n = mask.size
+    out = {"workload": "the FP32 line's pipe and initial state", "steps": steps}
+    for prec in ("f16s", "bf16s"):
+        with Solver(mask.shape, prm, mask=mask, precision=prec, kernel=KERNEL_AUTO) as s:
+            s.autotune(50)
+            s.set_state(*[a.astype(np.float32) for a in st])
+            s.step(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            ms = s.time_steps(steps) / steps
+            s.sync()
+            wall = time.perf_counter() - t0
+            B = algorithmic_bytes_per_cell(prec, True)
+            mlups = n * steps / wall / 1e6
+            out[prec] = {"mlups": round(mlups, 1), "kernel_ms": round(ms, 4),
+                         "chunk": s.tile_chunk,
+                         "roofline_frac": round(B * s.step_cells / (ms * 1e-3) / 1e9
+                                                / HBM_PEAK_GBS, 4),
+                         "vs_f32": round(mlups / f32_mlups, 3)}
+    return out
+
+
 def bench_tgv(n: int, steps: int, every: int, timed_steps: int, precisions):
     """Taylor-Green vortex on an n^3 periodic box (U0 = 0.05, Re 1600):
     step rate and roofline of the unmasked kernel, and the kinetic-energy
@@ -587,7 +615,7 @@ def main() -> int:
                                      not args.no_sequential)), flush=True)
         return 0
     if args.workload == "tgv":
-        print(json.dumps(bench_tgv(256, args.tgv_steps, 250, 100, ["f64", "f32", "f16s"])),
+        print(json.dumps(bench_tgv(256, args.tgv_steps, 250, 100, ["f64", "f32", "f16s", "bf16s"])),
               flush=True)
         return 0
     if args.workload == "stats":
@@ -687,10 +715,16 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mask, st, pin, nu, args.cpu_steps,
                                            args.precision)
+    if args.workload == "all" and args.precision == "f32":
+        try:
+            out["storage_modes"] = bench_storage_modes(mask, st, prm, args.steps, args.warmup,
+                                                       mlups)
+        except Exception as e:  # reported in the line, never hidden
+            out["storage_modes"] = {"error": f"{type(e).__name__}: {e}"}
     if args.workload == "all":
         for key, fn in (("small_pipe_f64", bench_small_pipe),
                         ("taylor_green", lambda: bench_tgv(256, 5000, 250, 100,
-                                                          ["f64", "f32", "f16s"])),
+                                                          ["f64", "f32", "f16s", "bf16s"])),
                         ("batched", lambda: bench_batch(args.batch_count, args.batch_steps,
                                                        not args.no_sequential)),
                         ("serving", lambda: bench_serve(args.serve_requests, args.serve_rate,
