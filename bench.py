@@ -560,7 +560,8 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=-1,
                     help="tiled kernel: z planes per work item (0 = library "
                          "default, -1 = autotune)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=25,
+                    help="steps of the CPU baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all",
                     choices=["all", "pipe", "batch", "serve", "stats", "tgv"],
