@@ -115,7 +115,7 @@ def lib() -> C.CDLL:
         "fslbm_set_tile_chunk": (i, [vp, i]),
         "fslbm_get_tile_chunk": (i, [vp]),
         "fslbm_step_cells": (C.c_longlong, [vp]),
-        "fslbm_tile_plan": (i, [i, i, i, i, C.c_void_p, ip, ip, C.POINTER(C.c_longlong), ip]),
+        "fslbm_tile_plan": (i, [i, i, i, i, d, C.c_void_p, ip, ip, C.POINTER(C.c_longlong), ip]),
         "fslbm_autotune": (i, [vp, i, ip, C.POINTER(d)]),
         "fslbm_set_state": (i, [vp, _dp, _dp, _dp, _dp]),
         "fslbm_get_state": (i, [vp, _dp, _dp, _dp, _dp]),

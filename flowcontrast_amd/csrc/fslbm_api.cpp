@@ -41,6 +41,7 @@ struct fslbm_ctx {
     int *tiles = nullptr;      // tiles holding fluid (device; nullptr: all)
     int ntiles = 0;
     long long tile_cells = 0;  // cells of those tiles
+    int rows = 0;              // rows of the tiles (plan_rows at create)
     int chunk = 0;             // tiled kernel z planes per work item
     double *partials = nullptr;  // reduction scratch
     int npart = 1024;
@@ -192,7 +193,7 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     if (c->kernel == FSLBM_KERNEL_TILED && !tiled_ok(c, c->prm.d1))
         return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
     if (use_tiled(c)) {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, c->chunk, c->stream};
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, c->chunk, c->rows, c->stream};
         e = launch_step_tiled(c->g, c->precision, in, out, aux, a);
     } else
         e = launch_step_plain(c->g, c->precision, in, out, c->mask, a, c->stream);
@@ -276,6 +277,7 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
     int rc = check_params(c, prm);
     if (rc) { g_create_err = c->err; delete c; return rc; }
     c->prm = *prm;
+    c->rows = plan_rows(precision, c->g.periodic != 0, prm->blend != 0.0);
     if (mask) {
         const long long nxy = (long long)nx * ny;
         for (long long i = 0; i < N; i++) {
@@ -348,13 +350,13 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
                      hipMemcpy(c->omega, om.data(), om.size() * sizeof(float),
                                hipMemcpyHostToDevice) == hipSuccess;
             }
-            const std::vector<int> tl = active_tiles(c->g, precision, mask);
+            const std::vector<int> tl = active_tiles(c->g, c->rows, mask);
             if (ok && !tl.empty()) {
                 ok = alloc((void **)&c->tiles, tl.size() * sizeof(int)) &&
                      hipMemcpy(c->tiles, tl.data(), tl.size() * sizeof(int),
                                hipMemcpyHostToDevice) == hipSuccess;
                 c->ntiles = (int)tl.size();
-                c->tile_cells = tiles_cells(c->g, precision, tl);
+                c->tile_cells = tiles_cells(c->g, c->rows, tl);
             }
         }
     }
@@ -433,7 +435,7 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
     a.q = c->q;
     a.gauge = 0.0;
     // the modelled best chunk lengths, then a few fixed ones
-    std::vector<int> cands = chunk_candidates(c->g, c->precision, c->ntiles, 6);
+    std::vector<int> cands = chunk_candidates(c->g, c->rows, c->ntiles, 6);
     for (int f : {32, 64, 128, 256})
         if (f <= c->g.nz && std::find(cands.begin(), cands.end(), f) == cands.end())
             cands.push_back(f);
@@ -441,12 +443,13 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
     double bms = 1e300;
     // untimed warm-up, so that the first candidate is not timed on an idle GPU
     {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, cands[0], c->stream};
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, cands[0], c->rows, c->stream};
         for (int t = 0; t < 2 * trial_steps; t++)
             HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
     }
     for (int cand : cands) {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, std::min(cand, c->g.nz), c->stream};
+        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, std::min(cand, c->g.nz), c->rows,
+                     c->stream};
         // warm-up launch, then timed launches
         HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -469,13 +472,15 @@ int fslbm_get_tile_chunk(fslbm_t c)
 {
     if (!c) return FSLBM_EINVAL;
     return c->chunk > 0 ? std::min(c->chunk, c->g.nz)
-                        : default_chunk(c->g, c->precision, c->ntiles);
+                        : default_chunk(c->g, c->rows, c->ntiles);
 }
 
-int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d,
-                    int *tile_rows, int *ntiles, long long *cells, int *chunk)
+int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
+                    const uint8_t *mask2d, int *tile_rows, int *ntiles, long long *cells,
+                    int *chunk)
 {
-    if (nx < 1 || ny < 1 || nz < 1 || precision < FSLBM_F64 || precision > FSLBM_BF16S)
+    if (nx < 1 || ny < 1 || nz < 1 || precision < FSLBM_F64 || precision > FSLBM_BF16S ||
+        !std::isfinite(blend))
         return FSLBM_EINVAL;
     Geo g;
     g.nx = nx;
@@ -483,13 +488,13 @@ int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d
     g.nz = nz;
     g.N = (long long)nx * ny * nz;
     g.periodic = mask2d == nullptr;
-    const int rows = tiled_rows(precision);
-    const std::vector<int> tl = active_tiles(g, precision, mask2d);
+    const int rows = plan_rows(precision, g.periodic != 0, blend != 0.0);
+    const std::vector<int> tl = active_tiles(g, rows, mask2d);
     const int all = ((nx + 63) / 64) * ((ny + rows - 1) / rows);
     if (tile_rows) *tile_rows = rows;
     if (ntiles) *ntiles = tl.empty() ? all : (int)tl.size();
-    if (cells) *cells = tl.empty() ? g.N : tiles_cells(g, precision, tl);
-    if (chunk) *chunk = default_chunk(g, precision, (int)tl.size());
+    if (cells) *cells = tl.empty() ? g.N : tiles_cells(g, rows, tl);
+    if (chunk) *chunk = default_chunk(g, rows, (int)tl.size());
     return FSLBM_OK;
 }
 

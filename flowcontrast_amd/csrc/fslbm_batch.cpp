@@ -386,7 +386,10 @@ int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, 
     encode_state(g, hm, b->precision, pd.q, src, pd.host.data());
     if (mask) {
         pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
-        pd.tiles = active_tiles(g, b->precision, pd.mask2d.data());
+        // planned with the rows of the domain's kernel class (batch_fill_dom
+        // and the class launch take the same)
+        pd.tiles = active_tiles(g, plan_rows(b->precision, false, prm->blend != 0.0),
+                                pd.mask2d.data());
     }
     return FSLBM_OK;
 }

@@ -62,13 +62,14 @@ hipError_t launch_step_plain(const Geo &g, int precision, const void *in,
 // the per-column wall weights omega (3 planes of nx*ny values in the
 // arithmetic type: sum over wall neighbours of W_e, W_e cx, W_e cy) and
 // the device list of tiles that hold a fluid cell (from active_tiles;
-// null: every tile).
+// null: every tile), and the rows of the tiles (plan_rows).
 struct TiledAux {
     const uint8_t *mask2d;
     const void *omega;
     const int *tiles;
     int ntiles;
     int chunk;  // z planes per work item (0: default)
+    int rows;
     hipStream_t stream;
 };
 hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
@@ -77,18 +78,22 @@ hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
 // Compute units of the current device.
 int device_cus();
-// Tile indices (ty * ntx + tx, in the tile shape of the precision mode)
-// whose cells include a fluid cell of the z-invariant mask plane; empty for
-// a periodic box (every tile).
-std::vector<int> active_tiles(const Geo &g, int precision, const uint8_t *mask2d);
+// Rows of the tiled kernels' tiles (64 columns wide): 8 for FP64 and BF16
+// storage, 12 for FP32, and for FP16 storage 12 in a walled box whose
+// corrector has no order-4 arms (blend 0), else 8.  A domain keeps the rows
+// it was planned with; a later blend change runs the other corrector
+// variant on the same tiles.
+int plan_rows(int precision, bool periodic, bool order4);
+// Tile indices (ty * ntx + tx, tiles of 64 x rows) whose cells include a
+// fluid cell of the z-invariant mask plane; empty for a periodic box
+// (every tile).
+std::vector<int> active_tiles(const Geo &g, int rows, const uint8_t *mask2d);
 // Lattice cells (all z planes) covered by those tiles, clipped to the box.
-long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles);
+long long tiles_cells(const Geo &g, int rows, const std::vector<int> &tiles);
 // Chunk lengths (z planes per work item) ordered by the modelled step
 // time over `ntiles` tiles (0: every tile); default_chunk is the first.
-std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count);
-int default_chunk(const Geo &g, int precision, int ntiles);
-// Rows of the tiled kernel's tiles (64 columns wide) in a precision mode.
-int tiled_rows(int precision);
+std::vector<int> chunk_candidates(const Geo &g, int rows, int ntiles, int count);
+int default_chunk(const Geo &g, int rows, int ntiles);
 
 // Flag on a batch work item's ze: the domain's lead item, whose workgroup
 // advances the domain's step counter.

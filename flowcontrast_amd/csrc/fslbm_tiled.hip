@@ -67,6 +67,10 @@ constexpr int NT = 704;           // 11 waves
 // robin; these counts keep the busiest SIMD's VALU load lowest (13 rows
 // or 9 rows put a fourth or third interior wave on one SIMD).
 constexpr int pipe_rows(int mode) { return mode == FSLBM_F32 ? 12 : 8; }
+// FP16 storage in a walled box without order-4 arms fits 12 rows (15
+// waves) in 128 registers too, and runs 6% faster there; its periodic and
+// order-4 variants spill at 12 rows and keep 8.
+constexpr int F16S_WALLED_ROWS = 12;
 
 template <typename R>
 struct Msg {
@@ -1136,11 +1140,10 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
                                            it.w & ~BATCH_LEAD, sr_key(seed));
 }
 
-template <int MODE>
-hipError_t launch_batch_mode(const BatchLaunch &b)
+template <int MODE, int TYP>
+hipError_t launch_batch_rows(const BatchLaunch &b)
 {
     using R = typename Store<MODE>::R;
-    constexpr int TYP = pipe_rows(MODE);
     const auto *doms = (const BatchDom<R> *)b.doms;
     const dim3 grid(b.grid), block((TYP + 3) * 64);
 #define FSLBM_BATCH(PER, DD, F2)                                                        \
@@ -1157,17 +1160,28 @@ hipError_t launch_batch_mode(const BatchLaunch &b)
     return hipGetLastError();
 }
 
+// A kernel class's tile rows follow from its precision, periodicity and
+// corrector variant, as the rows its domains' tiles were planned with.
+template <int MODE>
+hipError_t launch_batch_mode(const BatchLaunch &b)
+{
+    const int rows = plan_rows(MODE, b.periodic, b.fd2);
+    if constexpr (MODE == FSLBM_F16S)
+        if (rows == F16S_WALLED_ROWS) return launch_batch_rows<MODE, F16S_WALLED_ROWS>(b);
+    if (rows != pipe_rows(MODE)) return hipErrorInvalidValue;
+    return launch_batch_rows<MODE, pipe_rows(MODE)>(b);
+}
+
 // Host-side fill of the per-launch parameters of one domain.
 template <int MODE>
 void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const StepArgs &a,
-                 int chunk_req, const int *tiles, int ntiles)
+                 int chunk_req, const int *tiles, int ntiles, int rows)
 {
     using R = typename Store<MODE>::R;
     const StepK<R> k = [&] {
         if constexpr (MODE == FSLBM_F64) return make_consts_f64(a);
         else return make_consts_f32(a);
     }();
-    const int rows = MODE == FSLBM_F64 ? TY : pipe_rows(MODE);
     P.nx = g.nx;
     P.ny = g.ny;
     P.nz = g.nz;
@@ -1176,7 +1190,7 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.nty = (g.ny + rows - 1) / rows;
     P.tiles = tiles && ntiles > 0 ? tiles : nullptr;
     P.ntile = P.tiles ? ntiles : P.ntx * P.nty;
-    P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, MODE, P.ntile);
+    P.chunk = chunk_req > 0 ? chunk_req : default_chunk(g, rows, P.ntile);
     if (P.chunk > g.nz) P.chunk = g.nz;
     P.nchunks = (g.nz + P.chunk - 1) / P.chunk;
     P.cq = g.nz / P.nchunks;
@@ -1199,6 +1213,21 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.skey = k.skey;
 }
 
+template <int MODE, bool PERIODIC, int D1, int TYP>
+hipError_t launch_pipe(const void *in, void *out, const TiledAux &aux,
+                       const TiledParams<typename Store<MODE>::R> &P, int grid, hipStream_t s)
+{
+    using S = typename Store<MODE>::S;
+    using R = typename Store<MODE>::R;
+    if (P.c2 != R(0))
+        k_step_pipe<MODE, PERIODIC, D1, true, TYP><<<grid, (TYP + 3) * 64, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    else  // blend = 0: the order-4 arms have zero weight
+        k_step_pipe<MODE, PERIODIC, D1, false, TYP><<<grid, (TYP + 3) * 64, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    return hipGetLastError();
+}
+
 template <int MODE, bool PERIODIC, int D1>
 hipError_t launch_t(const Geo &g, const void *in, void *out,
                     const TiledAux &aux,
@@ -1213,14 +1242,14 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
     if constexpr (MODE == FSLBM_F64)
         k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
-    else if (P.c2 != R(0))
-        k_step_pipe<MODE, PERIODIC, D1, true, pipe_rows(MODE)>
-            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
-                (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
-    else  // blend = 0: the order-4 arms have zero weight
-        k_step_pipe<MODE, PERIODIC, D1, false, pipe_rows(MODE)>
-            <<<grid, (pipe_rows(MODE) + 3) * 64, 0, s>>>(
-                (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+    else if constexpr (MODE == FSLBM_F16S && !PERIODIC) {
+        if (aux.rows == F16S_WALLED_ROWS)
+            return launch_pipe<MODE, PERIODIC, D1, F16S_WALLED_ROWS>(in, out, aux, P, grid, s);
+    }
+    if constexpr (MODE != FSLBM_F64) {
+        if (aux.rows != pipe_rows(MODE)) return hipErrorInvalidValue;
+        return launch_pipe<MODE, PERIODIC, D1, pipe_rows(MODE)>(in, out, aux, P, grid, s);
+    }
     return hipGetLastError();
 }
 
@@ -1230,7 +1259,7 @@ hipError_t launch_mode(const Geo &g, const void *in, void *out,
 {
     using R = typename Store<MODE>::R;
     TiledParams<R> P;
-    fill_params<MODE>(P, g, a, aux.chunk, aux.tiles, aux.ntiles);
+    fill_params<MODE>(P, g, a, aux.chunk, aux.tiles, aux.ntiles, aux.rows);
     const bool per = g.periodic != 0;
     if (a.d1 == 1)
         return per ? launch_t<MODE, true, 1>(g, in, out, aux, P, aux.stream)
@@ -1278,13 +1307,17 @@ double chunk_cost(int nz, long long ntile, int nch, int cus)
     return (double)rounds * (len + ITEM_OVERHEAD);
 }
 
-int tile_rows(int precision) { return precision == FSLBM_F64 ? TY : pipe_rows(precision); }
-
 }  // namespace
 
-std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int count)
+int plan_rows(int precision, bool periodic, bool order4)
 {
-    const int rows = tile_rows(precision);
+    if (precision == FSLBM_F64) return TY;
+    if (precision == FSLBM_F16S && !periodic && !order4) return F16S_WALLED_ROWS;
+    return pipe_rows(precision);
+}
+
+std::vector<int> chunk_candidates(const Geo &g, int rows, int ntiles, int count)
+{
     const long long T = ntiles > 0 ? ntiles
                                    : (long long)((g.nx + TX - 1) / TX) * ((g.ny + rows - 1) / rows);
     const int cus = device_cus();
@@ -1311,18 +1344,15 @@ std::vector<int> chunk_candidates(const Geo &g, int precision, int ntiles, int c
     return out;
 }
 
-int tiled_rows(int precision) { return tile_rows(precision); }
-
-int default_chunk(const Geo &g, int precision, int ntiles)
+int default_chunk(const Geo &g, int rows, int ntiles)
 {
-    return chunk_candidates(g, precision, ntiles, 1)[0];
+    return chunk_candidates(g, rows, ntiles, 1)[0];
 }
 
-std::vector<int> active_tiles(const Geo &g, int precision, const uint8_t *mask2d)
+std::vector<int> active_tiles(const Geo &g, int rows, const uint8_t *mask2d)
 {
     std::vector<int> out;
     if (g.periodic || !mask2d) return out;
-    const int rows = tile_rows(precision);
     const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + rows - 1) / rows;
     for (int ty = 0; ty < nty; ty++)
         for (int tx = 0; tx < ntx; tx++) {
@@ -1337,9 +1367,8 @@ std::vector<int> active_tiles(const Geo &g, int precision, const uint8_t *mask2d
     return out;
 }
 
-long long tiles_cells(const Geo &g, int precision, const std::vector<int> &tiles)
+long long tiles_cells(const Geo &g, int rows, const std::vector<int> &tiles)
 {
-    const int rows = tile_rows(precision);
     const int ntx = (g.nx + TX - 1) / TX;
     long long n = 0;
     for (int t : tiles) {
@@ -1381,9 +1410,11 @@ int batch_fill_dom(void *dst, int precision, const Geo &g, const StepArgs &a, in
                    void *buf0, void *buf1, const uint8_t *mask2d, const void *omega,
                    const int *tiles, int ntiles, int parity, int split[3])
 {
+    // the domain's tiles were planned with these rows (batch_prepare)
+    const int rows = plan_rows(precision, g.periodic != 0, a.blend != 0.0);
     auto fill = [&](auto *D, auto tag) -> int {
         constexpr int M = decltype(tag)::value;
-        fill_params<M>(D->P, g, a, chunk, tiles, ntiles);
+        fill_params<M>(D->P, g, a, chunk, tiles, ntiles, rows);
         if (split) {
             split[0] = D->P.ntile;
             split[1] = D->P.cq;

@@ -215,11 +215,13 @@ class Solver:
         return out.reshape(11, ny, nx), s.value
 
 
-def tile_plan(shape, precision: str = "f32", mask=None) -> dict:
+def tile_plan(shape, precision: str = "f32", mask=None, blend: float = 0.0) -> dict:
     """The tiled kernel's schedule for a domain of shape (nz, ny, nx),
     computed on the host: tile rows, tiles holding fluid, cells one step
     moves and the default chunk.  `mask` is the z-invariant mask (one
-    (ny, nx) plane or the full volume), None for a periodic box."""
+    (ny, nx) plane or the full volume), None for a periodic box; `blend`
+    the corrector's order-4 weight the domain is created with (FP16
+    storage plans 12-row tiles in a walled box at blend 0, else 8)."""
     nz, ny, nx = shape
     mptr = None
     if mask is not None:
@@ -227,7 +229,8 @@ def tile_plan(shape, precision: str = "f32", mask=None) -> dict:
         mptr = m.ctypes.data_as(C.c_void_p)
     rows, ntiles, chunk = C.c_int(), C.c_int(), C.c_int()
     cells = C.c_longlong()
-    check(lib().fslbm_tile_plan(nx, ny, nz, _lib.PRECISIONS[precision], mptr, C.byref(rows),
-                                C.byref(ntiles), C.byref(cells), C.byref(chunk)))
+    check(lib().fslbm_tile_plan(nx, ny, nz, _lib.PRECISIONS[precision], float(blend), mptr,
+                                C.byref(rows), C.byref(ntiles), C.byref(cells),
+                                C.byref(chunk)))
     return {"tile_rows": rows.value, "tiles": ntiles.value, "cells": cells.value,
             "chunk": chunk.value}

@@ -112,14 +112,18 @@ long long fslbm_step_cells(fslbm_t ctx);
 
 /* Host-side plan of the tiled kernel's schedule for a domain (no reference
  * counterpart; no device needed, for capacity planning and tests).
+ * blend: the corrector's order-4 weight the domain is created with.
  * mask2d: one z plane of nx*ny codes (0/125), or NULL for a periodic box.
- * Outputs, any of which may be NULL: the rows of the mode's tiles (64
- * columns wide), the number of tiles holding fluid (work items run on
- * these only), the cells one step moves (as fslbm_step_cells) and the
- * default chunk (z planes per work item).  FP64 runs the two-barrier
- * kernel on 8-row tiles. */
-int fslbm_tile_plan(int nx, int ny, int nz, int precision, const uint8_t *mask2d,
-                    int *tile_rows, int *ntiles, long long *cells, int *chunk);
+ * Outputs, any of which may be NULL: the rows of the tiles (64 columns
+ * wide; 12 for FP32, 8 for FP64 and BF16S, and for FP16S 12 in a walled
+ * box created with blend 0, else 8), the number of tiles holding fluid
+ * (work items run on these only), the cells one step moves (as
+ * fslbm_step_cells) and the default chunk (z planes per work item).  A
+ * context keeps the tiles it was created with when its blend changes.
+ * FP64 runs the two-barrier kernel on 8-row tiles. */
+int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
+                    const uint8_t *mask2d, int *tile_rows, int *ntiles, long long *cells,
+                    int *chunk);
 
 /* Autotuner of the tiled kernel's schedule: after 2*trial_steps untimed
  * launches, times trial_steps launches for each candidate chunk (the

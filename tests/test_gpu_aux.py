@@ -246,6 +246,31 @@ def test_autotune_keeps_state_and_results():
         assert np.array_equal(a, b)
 
 
+def test_f16s_tiles_follow_the_creation_blend():
+    """FP16 storage in a walled box plans 12-row tiles at blend 0 and 8-row
+    tiles otherwise; a context keeps its tiles when its blend changes.  The
+    tile shape changes only the schedule: a context created at blend 0 and
+    switched to 0.5 gives bitwise the state of one created at 0.5."""
+    from flowcontrast_amd import tile_plan
+    nx, ny, nz, R = 70, 60, 40, 26.0
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    st = [a.astype(np.float32) for a in inputs.poiseuille_state(mask, R, 0.04, 1e-3, 0.05, 4)]
+    prm0 = RunParams(tblend=0.9, pin=1e-3, nu=0.01, d1=2, delta=1, blend=0.0)
+    prm5 = RunParams(tblend=0.9, pin=1e-3, nu=0.01, d1=2, delta=1, blend=0.5)
+    assert tile_plan(mask.shape, "f16s", mask, 0.0)["tile_rows"] == 12
+    assert tile_plan(mask.shape, "f16s", mask, 0.5)["tile_rows"] == 8
+    outs = []
+    for first in (prm0, prm5):
+        with Solver(mask.shape, first, mask=mask, precision="f16s", kernel=KERNEL_TILED) as s:
+            assert s.step_cells == tile_plan(mask.shape, "f16s", mask, first.blend)["cells"]
+            s.set_state(*st)
+            s.set_params(prm5)
+            s.step(6)
+            outs.append(s.get_state(np.float32))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("precision", ["f16s", "bf16s"])
 @pytest.mark.parametrize("variant", _variants())
 def test_shifted_storage_rounding_is_unbiased(precision, variant):

@@ -204,7 +204,7 @@ def test_wall_only_tiles_are_skipped_exactly(orc, precision, tol):
         s.set_gauge(5e-4)
         s.step(7)
         got = s.get_state()
-        plan = tile_plan(mask.shape, precision, mask)
+        plan = tile_plan(mask.shape, precision, mask, prm.blend)
         assert s.step_cells == plan["cells"] < mask.size
         assert s.tile_chunk == plan["chunk"]
     wall = mask != 0

@@ -9,7 +9,17 @@ import pytest
 from flowcontrast_amd import inputs, tile_plan
 from flowcontrast_amd._lib import FslbmError
 
-ROWS = {"f64": 8, "f32": 12, "f16s": 8, "bf16s": 8}
+PRECISIONS = ("f64", "f32", "f16s", "bf16s")
+
+
+def rows_of(prec, periodic=False, blend=0.0):
+    """Tile rows: 12 for FP32, 8 for FP64 and BF16 storage; FP16 storage
+    takes 12 in a walled box at blend 0 and 8 otherwise."""
+    if prec == "f32":
+        return 12
+    if prec == "f16s" and not periodic and blend == 0.0:
+        return 12
+    return 8
 
 
 def brute_tiles(mask2d, rows):
@@ -46,18 +56,21 @@ def model_chunk(nz, ntile):
 
 
 def test_pipe_r120_cross_section():
-    """The 256x256 cross-section of the R = 120 pipe: 78 of 88 FP32 tiles
-    (64 x 12) and 112 of 128 16-bit tiles (64 x 8) hold fluid."""
+    """The 256x256 cross-section of the R = 120 pipe: 78 of 88 tiles of
+    64 x 12 and 112 of 128 tiles of 64 x 8 hold fluid."""
     mask, _ = inputs.circle_mask(256, 256, 1, 120.0)
     nz = 1024
-    for prec, tiles in (("f32", 78), ("f16s", 112), ("bf16s", 112), ("f64", 112)):
-        plan = tile_plan((nz, 256, 256), prec, mask)
-        assert plan["tile_rows"] == ROWS[prec]
+    for prec, blend, tiles in (("f32", 0.0, 78), ("f16s", 0.0, 78), ("f16s", 1.0, 112),
+                               ("bf16s", 0.0, 112), ("f64", 0.0, 112)):
+        plan = tile_plan((nz, 256, 256), prec, mask, blend)
+        rows = rows_of(prec, blend=blend)
+        assert plan["tile_rows"] == rows
         assert plan["tiles"] == tiles
-        assert plan["cells"] == tiles * 64 * ROWS[prec] * nz
+        assert plan["cells"] == tiles * 64 * rows * nz
         assert plan["chunk"] == model_chunk(nz, tiles)
     assert tile_plan((nz, 256, 256), "f32", mask)["chunk"] == 79
-    assert tile_plan((nz, 256, 256), "f16s", mask)["chunk"] == 114
+    assert tile_plan((nz, 256, 256), "f16s", mask)["chunk"] == 79
+    assert tile_plan((nz, 256, 256), "bf16s", mask)["chunk"] == 114
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
@@ -73,20 +86,24 @@ def test_random_masks_match_brute_force(seed):
             m[((x - cx) ** 2 + (y - cy) ** 2 <= r * r) & (x > 0) & (y > 0)
               & (x < nx - 1) & (y < ny - 1)] = 0
         nz = int(rng.integers(16, 80))
-        for prec in ROWS:
-            plan = tile_plan((nz, ny, nx), prec, m)
-            tiles, cells = brute_tiles(m, ROWS[prec])
-            if tiles == 0:  # no fluid: one item, which stores nothing
-                tiles, cells = 1, min(64, nx) * min(ROWS[prec], ny)
-            assert plan["tiles"] == tiles, (nx, ny, prec)
-            assert plan["cells"] == cells * nz, (nx, ny, prec)
-            assert plan["chunk"] == model_chunk(nz, tiles), (nx, ny, nz, prec)
+        for prec in PRECISIONS:
+            for blend in (0.0, 0.5):
+                plan = tile_plan((nz, ny, nx), prec, m, blend)
+                rows = rows_of(prec, blend=blend)
+                assert plan["tile_rows"] == rows
+                tiles, cells = brute_tiles(m, rows)
+                if tiles == 0:  # no fluid: one item, which stores nothing
+                    tiles, cells = 1, min(64, nx) * min(rows, ny)
+                assert plan["tiles"] == tiles, (nx, ny, prec)
+                assert plan["cells"] == cells * nz, (nx, ny, prec)
+                assert plan["chunk"] == model_chunk(nz, tiles), (nx, ny, nz, prec)
 
 
 def test_periodic_box_steps_every_tile():
-    for prec in ROWS:
+    for prec in PRECISIONS:
         plan = tile_plan((40, 88, 88), prec, None)
-        rows = ROWS[prec]
+        rows = rows_of(prec, periodic=True)
+        assert plan["tile_rows"] == rows
         assert plan["tiles"] == 2 * -(-88 // rows)
         assert plan["cells"] == 88 * 88 * 40
 
@@ -113,3 +130,5 @@ def test_chunk_model_shape():
 def test_invalid_arguments():
     with pytest.raises(FslbmError):
         tile_plan((0, 32, 32), "f32", None)
+    with pytest.raises(FslbmError):
+        tile_plan((8, 32, 32), "f32", None, float("nan"))
