@@ -5,9 +5,10 @@
 //
 // * One workgroup owns a 64 x TY (x, y) tile and marches over a chunk of
 //   z planes: TY = 8 in the two-barrier FP64 kernel (k_step_tiled), 12
-//   (FP32) or 8 (16-bit storage) in the pipelined kernel (k_step_pipe,
-//   further below).  Every state value is written once per step and read
-//   from HBM about once (halo and chunk-end re-reads mostly hit L2).
+//   (FP32, and FP16 storage in a walled box at blend 0) or 8 (other 16-bit
+//   storage) in the pipelined kernel (k_step_pipe, further below).  Every
+//   state value is written once per step and read from HBM about once
+//   (1.04-1.07x: halo and chunk-end re-reads mostly hit L2).
 // * The 27-point D3Q27 predictor is evaluated as three separable 3-tap
 //   passes.  D3Q27 weights are products w(cx) w(cy) w(cz) with
 //   w(0) = 2/3, w(+-1) = 1/6, and the equilibrium
