@@ -277,7 +277,7 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
     int rc = check_params(c, prm);
     if (rc) { g_create_err = c->err; delete c; return rc; }
     c->prm = *prm;
-    c->rows = plan_rows(precision, c->g.periodic != 0, prm->blend != 0.0);
+    c->rows = solo_rows(precision, c->g, prm->blend != 0.0);
     if (mask) {
         const long long nxy = (long long)nx * ny;
         for (long long i = 0; i < N; i++) {
@@ -488,7 +488,7 @@ int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
     g.nz = nz;
     g.N = (long long)nx * ny * nz;
     g.periodic = mask2d == nullptr;
-    const int rows = plan_rows(precision, g.periodic != 0, blend != 0.0);
+    const int rows = solo_rows(precision, g, blend != 0.0);
     const std::vector<int> tl = active_tiles(g, rows, mask2d);
     const int all = ((nx + 63) / 64) * ((ny + rows - 1) / rows);
     if (tile_rows) *tile_rows = rows;

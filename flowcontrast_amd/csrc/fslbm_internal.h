@@ -84,6 +84,11 @@ int device_cus();
 // it was planned with; a later blend change runs the other corrector
 // variant on the same tiles.
 int plan_rows(int precision, bool periodic, bool order4);
+// Rows of a single domain's tiles: plan_rows, except that an FP32 periodic
+// box (which steps every tile) takes 8 rows where the step-time model
+// rates their best schedule faster than that of 12 (batches keep 12: one
+// launch per kernel class takes one tile shape).
+int solo_rows(int precision, const Geo &g, bool order4);
 // Tile indices (ty * ntx + tx, tiles of 64 x rows) whose cells include a
 // fluid cell of the z-invariant mask plane; empty for a periodic box
 // (every tile).
