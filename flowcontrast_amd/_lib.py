@@ -23,6 +23,7 @@ EXPORTS = [
     "fslbm_set_kernel", "fslbm_get_kernel", "fslbm_set_tile_chunk",
     "fslbm_get_tile_chunk", "fslbm_step_cells", "fslbm_tile_plan", "fslbm_autotune", "fslbm_set_state",
     "fslbm_get_state", "fslbm_set_state_f32", "fslbm_get_state_f32",
+    "fslbm_snapshot_size", "fslbm_snapshot_save", "fslbm_snapshot_load",
     "fslbm_step", "fslbm_set_gauge", "fslbm_monitor", "fslbm_reduce",
     "fslbm_probe", "fslbm_get_plane", "fslbm_stats_config",
     "fslbm_stats_accumulate", "fslbm_stats_read", "fslbm_stats_load",
@@ -121,6 +122,9 @@ def lib() -> C.CDLL:
         "fslbm_get_state": (i, [vp, _dp, _dp, _dp, _dp]),
         "fslbm_set_state_f32": (i, [vp, _fp, _fp, _fp, _fp]),
         "fslbm_get_state_f32": (i, [vp, _fp, _fp, _fp, _fp]),
+        "fslbm_snapshot_size": (C.c_longlong, [vp]),
+        "fslbm_snapshot_save": (i, [vp, vp, C.c_longlong]),
+        "fslbm_snapshot_load": (i, [vp, vp, C.c_longlong]),
         "fslbm_step": (i, [vp, i]),
         "fslbm_set_gauge": (i, [vp, d]),
         "fslbm_monitor": (i, [vp, i, _dp]),

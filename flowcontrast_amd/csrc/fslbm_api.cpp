@@ -532,6 +532,79 @@ int fslbm_get_state_f32(fslbm_t c, float *p, float *ux, float *uy, float *uz)
     return get_state_impl(c, p, ux, uy, uz);
 }
 
+namespace {
+
+// A snapshot: this header, then the stored state as it sits on the device
+// (4 fields of N values, storage_bytes each).
+struct SnapHeader {
+    char magic[8];
+    int32_t precision, pad;
+    int64_t n;        // lattice cells
+    Quant q;          // scales of the 16-bit storage modes
+    uint32_t steps;   // stochastic-rounding step counter
+    uint32_t pad2;
+    double gauge;     // pending gauge of the next step
+};
+constexpr char SNAP_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'S', 'N', '1'};
+
+}  // namespace
+
+long long fslbm_snapshot_size(fslbm_t c)
+{
+    if (!c) return FSLBM_EINVAL;
+    return (long long)(sizeof(SnapHeader) + storage_bytes(c->precision) * 4 * (size_t)c->g.N);
+}
+
+int fslbm_snapshot_save(fslbm_t c, void *buf, long long bytes)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!buf) return fail(c, FSLBM_EINVAL, "NULL snapshot buffer");
+    const long long need = fslbm_snapshot_size(c);
+    if (bytes < need)
+        return fail(c, FSLBM_EINVAL, "snapshot buffer of %lld bytes, %lld needed", bytes, need);
+    SnapHeader h{};
+    std::memcpy(h.magic, SNAP_MAGIC, 8);
+    h.precision = c->precision;
+    h.n = c->g.N;
+    h.q = c->q;
+    h.steps = c->steps;
+    h.gauge = c->gauge;
+    std::memcpy(buf, &h, sizeof h);
+    HIPCHK(c, hipMemcpyAsync((char *)buf + sizeof h, c->state[c->cur], (size_t)need - sizeof h,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FSLBM_OK;
+}
+
+int fslbm_snapshot_load(fslbm_t c, const void *buf, long long bytes)
+{
+    if (!c) return FSLBM_EINVAL;
+    if (!buf) return fail(c, FSLBM_EINVAL, "NULL snapshot buffer");
+    SnapHeader h;
+    if (bytes < (long long)sizeof h) return fail(c, FSLBM_EINVAL, "snapshot too short");
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, SNAP_MAGIC, 8) != 0) return fail(c, FSLBM_EINVAL, "not a snapshot");
+    const long long need = fslbm_snapshot_size(c);
+    if (h.precision != c->precision || h.n != c->g.N || bytes < need)
+        return fail(c, FSLBM_EINVAL, "snapshot of precision %d and %lld cells (%lld bytes) for "
+                    "a context of precision %d and %lld cells", (int)h.precision,
+                    (long long)h.n, bytes, c->precision, (long long)c->g.N);
+    if (!std::isfinite(h.gauge)) return fail(c, FSLBM_EINVAL, "non-finite gauge in snapshot");
+    // both buffers, as fslbm_set_state: wall cells hold zero in each
+    const size_t sb = (size_t)need - sizeof h;
+    const char *src = (const char *)buf + sizeof h;
+    hipError_t e = hipMemcpyAsync(c->state[c->cur], src, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->state[c->cur ^ 1], src, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess)
+        return fail(c, FSLBM_EHIP, "snapshot upload failed: %s", hipGetErrorString(e));
+    c->q = h.q;
+    c->steps = h.steps;
+    c->gauge = h.gauge;
+    return FSLBM_OK;
+}
+
 int fslbm_step(fslbm_t c, int nsteps)
 {
     if (!c) return FSLBM_EINVAL;

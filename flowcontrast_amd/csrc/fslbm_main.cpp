@@ -15,11 +15,13 @@
 //        p3Dii ... _avii.txt, qpii.txt backup every 50000 steps (:825-904)
 //        fslbm_ckpt.bin  exact checkpoint written with each backup: the
 //                   state and the statistics sums as doubles, the step
-//                   number and the pending gauge.  A run started without
+//                   number, the pending gauge and the context's exact
+//                   snapshot (fslbm_snapshot_save).  A run started without
 //                   --fresh resumes from it when present (the reference's
 //                   float32 volumes and %e averages are the fallback), and
 //                   continues the step numbering, so an interrupted run
-//                   repeats a continuous one bitwise (FP64 and FP32 modes).
+//                   repeats a continuous one bitwise in every precision
+//                   mode (a resume in another mode starts from the doubles).
 // The geometry sizes, hard-coded in the reference (FSLBM.cu:512-514), are
 // options here.  param.txt is re-read every 500 steps (FSLBM.cu:669-677).
 // The statistics and plane files are read from the device at their step
@@ -217,11 +219,20 @@ void write_checkpoint(fslbm_t ctx, int nx, int ny, int nz, int precision, long s
     check(fslbm_stats_read(ctx, acc.data(), &cnt, &cnt2), ctx, "checkpoint stats");
     h.cnt = cnt;
     h.cnt2 = cnt2;
+    // the exact snapshot (stored state, 16-bit scales, rounding counter)
+    // follows, so a resume in the same precision mode continues bitwise
+    const long long sn = fslbm_snapshot_size(ctx);
+    if (sn < 0) die("checkpoint snapshot size");
+    std::vector<char> snap((size_t)sn);
+    check(fslbm_snapshot_save(ctx, snap.data(), sn), ctx, "checkpoint snapshot");
+    const int64_t sn64 = sn;
     FILE *f = std::fopen("fslbm_ckpt.bin.tmp", "wb");
     if (!f) die("cannot write fslbm_ckpt.bin");
     bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
     for (auto &v : st) ok = ok && std::fwrite(v.data(), sizeof(double), N, f) == N;
     ok = ok && std::fwrite(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    ok = ok && std::fwrite(&sn64, sizeof sn64, 1, f) == 1 &&
+         std::fwrite(snap.data(), 1, snap.size(), f) == snap.size();
     ok = std::fclose(f) == 0 && ok;
     if (!ok || std::rename("fslbm_ckpt.bin.tmp", "fslbm_ckpt.bin") != 0)
         die("writing fslbm_ckpt.bin failed");
@@ -255,10 +266,21 @@ long read_checkpoint(fslbm_t ctx, int nx, int ny, int nz, int precision)
     if (h.stats_n != fslbm_stats_points(ctx)) ok = false;
     std::vector<double> acc(11 * (size_t)(h.stats_n > 0 ? h.stats_n : 0));
     ok = ok && std::fread(acc.data(), sizeof(double), acc.size(), f) == acc.size();
+    // the exact snapshot, when the file has one for this precision mode
+    std::vector<char> snap;
+    int64_t sn = 0;
+    if (ok && h.precision == precision && std::fread(&sn, sizeof sn, 1, f) == 1 &&
+        sn == fslbm_snapshot_size(ctx)) {
+        snap.resize((size_t)sn);
+        if (std::fread(snap.data(), 1, snap.size(), f) != snap.size()) snap.clear();
+    }
     std::fclose(f);
     if (!ok) die("fslbm_ckpt.bin: truncated");
     check(fslbm_set_state(ctx, st[0].data(), st[1].data(), st[2].data(), st[3].data()), ctx,
           "checkpoint state");
+    if (!snap.empty())
+        check(fslbm_snapshot_load(ctx, snap.data(), (long long)snap.size()), ctx,
+              "checkpoint snapshot");
     check(fslbm_stats_load(ctx, acc.data(), h.cnt, h.cnt2), ctx, "checkpoint stats");
     if (h.gauge != 0.0) check(fslbm_set_gauge(ctx, h.gauge), ctx, "checkpoint gauge");
     return (long)h.step;

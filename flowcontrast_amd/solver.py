@@ -137,6 +137,20 @@ class Solver:
             self._chk(lib().fslbm_get_state(self._h, *out))
         return [a.reshape(self.shape) for a in out]
 
+    def snapshot(self) -> bytes:
+        """Exact snapshot of the stepping state (stored state, 16-bit
+        storage scales, stochastic-rounding step counter, pending gauge)."""
+        n = self._chk(lib().fslbm_snapshot_size(self._h))
+        buf = C.create_string_buffer(n)
+        self._chk(lib().fslbm_snapshot_save(self._h, buf, n))
+        return buf.raw
+
+    def load_snapshot(self, snap: bytes) -> None:
+        """Restore a snapshot of a context with the same precision and
+        cells: stepping then continues bitwise as from where it was taken."""
+        buf = C.create_string_buffer(bytes(snap), len(snap))
+        self._chk(lib().fslbm_snapshot_load(self._h, buf, len(snap)))
+
     # -- stepping ---------------------------------------------------------------
     def step(self, nsteps: int = 1) -> None:
         self._chk(lib().fslbm_step(self._h, int(nsteps)))

@@ -149,6 +149,18 @@ int fslbm_set_state_f32(fslbm_t ctx, const float *p, const float *ux,
 int fslbm_get_state_f32(fslbm_t ctx, float *p, float *ux, float *uy,
                         float *uz);
 
+/* Exact snapshot of a context's stepping state (no reference counterpart;
+ * for checkpoints that resume bitwise in every precision mode): the state
+ * as stored on the device, the scales of the 16-bit storage modes, the
+ * stochastic-rounding step counter and the pending gauge.  A state set
+ * from doubles is re-quantised and restarts that counter, so only a
+ * snapshot resumes 16-bit storage exactly.  size: the snapshot's bytes;
+ * save fills buf (at least that many bytes, after the queued steps end);
+ * load takes a snapshot of a context with the same precision and cells. */
+long long fslbm_snapshot_size(fslbm_t ctx);
+int fslbm_snapshot_save(fslbm_t ctx, void *buf, long long bytes);
+int fslbm_snapshot_load(fslbm_t ctx, const void *buf, long long bytes);
+
 /* Advance nsteps time steps: predictor + CCS corrector + blend + periodic
  * and wall boundaries (FSLBM.cu:622-628).  A pending gauge (set by
  * fslbm_monitor or fslbm_set_gauge) is subtracted from the new pressure of

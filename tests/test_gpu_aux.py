@@ -246,6 +246,36 @@ def test_autotune_keeps_state_and_results():
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s", "bf16s"])
+def test_snapshot_resumes_bitwise(precision):
+    """A snapshot taken mid-run and loaded into a fresh context continues
+    bitwise as the original; a state set from doubles does not resume the
+    16-bit modes exactly (re-quantised, rounding counter restarted)."""
+    nx, ny, nz, R = 40, 36, 24, 15.0
+    mask, _ = inputs.circle_mask(nx, ny, nz, R)
+    st = inputs.poiseuille_state(mask, R, 0.05, 2e-3, 0.05, 6)
+    prm = RunParams(tblend=0.9, pin=2e-3, nu=0.01, d1=2, delta=1, blend=0.0)
+    with Solver(mask.shape, prm, mask=mask, precision=precision) as a:
+        a.set_state(*st)
+        a.step(7)
+        a.set_gauge(1e-4)
+        snap = a.snapshot()
+        a.step(5)
+        ref = a.get_state()
+    with Solver(mask.shape, prm, mask=mask, precision=precision) as b:
+        b.set_state(*st)
+        b.load_snapshot(snap)
+        b.step(5)
+        got = b.get_state()
+        for g, e in zip(got, ref):
+            assert np.array_equal(g, e)
+        with pytest.raises(FslbmError):  # truncated
+            b.load_snapshot(snap[:-1])
+    with Solver(mask.shape, prm, mask=mask, precision="f32" if precision != "f32" else "f64") as c:
+        with pytest.raises(FslbmError):  # another precision mode
+            c.load_snapshot(snap)
+
+
 def test_f16s_tiles_follow_the_creation_blend():
     """FP16 storage in a walled box plans 12-row tiles at blend 0 and 8-row
     tiles otherwise; a context keeps its tiles when its blend changes.  The

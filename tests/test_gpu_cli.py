@@ -117,11 +117,12 @@ def test_cli_backup_and_restart(tmp_path):
     assert int(cnt1[0]) == int(cnt0[0]) + 1
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f16s", "bf16s"])
 def test_cli_exact_checkpoint_resumes_bitwise(tmp_path, precision):
     """A run interrupted at a backup and resumed from fslbm_ckpt.bin writes
     the same files as an uninterrupted run (state, statistics sums, step
-    number and pending gauge are restored exactly)."""
+    number and pending gauge are restored exactly; in the 16-bit storage
+    modes also the stored values, their scales and the rounding counter)."""
     a, b = tmp_path / "a", tmp_path / "b"
     a.mkdir()
     b.mkdir()
