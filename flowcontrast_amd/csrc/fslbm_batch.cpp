@@ -54,6 +54,7 @@ struct fslbm_batch {
     int d1 = 2;
     size_t budget = 0, used = 0;
     std::vector<Domain> doms;
+    bool nt = false;  // non-temporal FP32 stores: the stepped state exceeds the LLC
     void *d_doms = nullptr;
     size_t d_doms_cap = 0;
     uint32_t *d_seeds = nullptr;  // two step counters per domain slot
@@ -195,9 +196,11 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     std::vector<uint32_t> seeds(2 * std::max<size_t>(nd, 1), 0u);
     const int ch = chunk_override >= 0 ? chunk_override : b->chunk;
     std::vector<Column> cols[NCLASS];
+    size_t stepped = 0;  // state bytes the step reads and writes
     for (size_t i = 0; i < nd; i++) {
         Domain &d = b->doms[i];
         if (!d.active || d.held) continue;
+        stepped += 2 * state_bytes(d.g, b->precision);
         // the current parity of each domain is folded into its descriptor
         d.parity ^= b->flip;
         int split[3] = {1, 0, 0};
@@ -213,6 +216,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         else
             for (int t : d.htiles) cl.push_back({(int)i, t, 0, d.g.nz});
     }
+    b->nt = stream_stores(stepped);
     const int cus = device_cus();
     std::vector<int4> work[NCLASS];
     for (int c = 0; c < NCLASS; c++) work[c] = guided_split(cols[c], ch > 0 ? ch : 128, cus, nd);
@@ -268,6 +272,7 @@ hipError_t launch_step_all(fslbm_batch *b, int flip)
         L.grid = b->nwork[c];  // one workgroup per item
         L.periodic = (c & 2) != 0;
         L.fd2 = (c & 1) != 0;
+        L.nt = b->nt;
         L.d1 = b->d1;
         L.precision = b->precision;
         L.stream = b->stream;

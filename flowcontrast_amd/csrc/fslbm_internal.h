@@ -52,6 +52,17 @@ StepK<double> make_consts_f64(const StepArgs &a);
 StepK<float> make_consts_f32(const StepArgs &a);
 
 size_t storage_bytes(int precision);  // bytes per stored value
+// Bytes of one state buffer (four fields) of a domain.
+inline size_t state_bytes(const Geo &g, int precision)
+{
+    return storage_bytes(precision) * 4 * (size_t)g.N;
+}
+// Last-level (Infinity) cache of the GPU: 256 MB on MI355X.
+constexpr size_t LLC_BYTES = size_t(256) << 20;
+// Whether FP32 steps store the new state non-temporal: only when the
+// state one step reads and writes (`bytes`, both buffers) exceeds the
+// last-level cache, which otherwise keeps it for the next step.
+inline bool stream_stores(size_t bytes) { return bytes > LLC_BYTES; }
 
 // Kernel launchers (fslbm_kernels.hip, fslbm_tiled.hip).  `in`/`out` are
 // the base of four SoA planes of N stored values each.
@@ -117,6 +128,7 @@ struct BatchLaunch {
     const int4 *work;  // (domain, tile, zs, ze | BATCH_LEAD)
     int nwork, flip, grid;
     bool periodic, fd2;
+    bool nt;  // FP32 non-temporal stores (stream_stores over the batch)
     int d1, precision;
     hipStream_t stream;
 };

@@ -246,22 +246,29 @@ __device__ __forceinline__ uint16_t bload<uint16_t>(__amdgpu_buffer_rsrc_t r, un
 {
     return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
 }
-// FP32 state is stored non-temporal (cache-policy bit NT): nothing in
-// this launch reads it back, so its lines should not displace the input
-// planes that neighbouring tiles still read from L2 (halo rows and
-// columns).  Measured +4.6% on the FP32 step of a 2 GB pipe and +6% on the
-// 256^3 Taylor-Green box.  16-bit stores stay ordinary: with NT they wrote
-// 8% more bytes and the FP16-storage Taylor-Green step ran 10% slower.
-// Non-temporal loads lose the halo sharing and measured 30% slower.
+// With NT, FP32 state is stored non-temporal (cache-policy bit NT):
+// nothing in the launch reads it back, so its lines do not displace the
+// input planes that neighbouring tiles still read from L2 (halo rows and
+// columns).  That pays only when the two state buffers exceed the 256 MB
+// Infinity Cache (stream_stores): then +4.5% on a 2 GB pipe and +6% on a
+// 393 MB one.  Below it, ordinary stores leave the new state in that cache
+// for the next step to read, and NT measured 35% slower (138 MB pipe) and
+// 22% slower (226 MB Taylor-Green box).  16-bit stores stay ordinary: with
+// NT they wrote 8% more bytes and the FP16-storage Taylor-Green step ran
+// 10% slower.  Non-temporal loads lose the halo sharing: 30% slower.
 constexpr int STORE_NT = 2;
+template <bool NT>
 __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, STORE_NT);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff,
+                                          NT ? STORE_NT : 0);
 }
+template <bool NT>
 __device__ __forceinline__ void bstore(_Float16 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
 }
+template <bool NT>
 __device__ __forceinline__ void bstore(uint16_t v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
     __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v, r, voff, soff, 0);
@@ -597,7 +604,7 @@ __device__ __forceinline__ float lane_shift(float src, float edge)
 // One work item of the pipelined step: tile `tile` (ty * ntx + tx) over
 // the z planes [zs, ze).  Called by the single-domain kernel and by the
 // batched kernel; all threads of the workgroup pass the same item.
-template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS>
 __device__ __forceinline__ void pipe_item(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
@@ -854,10 +861,10 @@ __device__ __forceinline__ void pipe_item(
                 const S z0 = S(0);
                 const SrWords rw = ST::SR ? sr_words((uint32_t)c * (uint32_t)nxy + col, skey)
                                           : SrWords{0u, 0u};
-                bstore(store ? ST::st_p_sr(np_, c, P.q, rw.a) : z0, oP, sb_, so);
-                bstore(store ? ST::st_u_sr(nx_, P.q, rw.a << 16) : z0, oU, sb_, so);
-                bstore(store ? ST::st_u_sr(ny_, P.q, rw.b) : z0, oV, sb_, so);
-                bstore(store ? ST::st_u_sr(nz_, P.q, rw.b << 16) : z0, oW, sb_, so);
+                bstore<NTS>(store ? ST::st_p_sr(np_, c, P.q, rw.a) : z0, oP, sb_, so);
+                bstore<NTS>(store ? ST::st_u_sr(nx_, P.q, rw.a << 16) : z0, oU, sb_, so);
+                bstore<NTS>(store ? ST::st_u_sr(ny_, P.q, rw.b) : z0, oV, sb_, so);
+                bstore<NTS>(store ? ST::st_u_sr(nz_, P.q, rw.b << 16) : z0, oW, sb_, so);
             }
             if (doB) {  // stage B: plane a-1; corrector tile of a-1
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
@@ -1081,7 +1088,7 @@ __device__ __forceinline__ void pipe_item(
     }
 }
 
-template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS>
 __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
@@ -1095,7 +1102,8 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     if (w >= P.nwork) return;
     int tile, zs, ze;
     item_of(P, w, tile, zs, ze);
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP>(in, out, mask2d, omega, P, tile, zs, ze, P.skey);
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS>(in, out, mask2d, omega, P, tile, zs, ze,
+                                                P.skey);
 }
 
 // ---------------------------------------------------------------------
@@ -1124,7 +1132,7 @@ struct BatchDom {
 // table) writes seeds[2d + (f ^ 1)] = that + 1 for the next launch (no
 // launch reads and writes the same slot), so hipGraph replays of the two
 // launches keep counting.
-template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP>
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS>
 __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const BatchDom<typename Store<MODE>::R> *__restrict__ doms,
     uint32_t *__restrict__ seeds, const int4 *__restrict__ work, int nwork, int flip)
@@ -1143,19 +1151,19 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const int par = D.parity ^ flip;
     const uint32_t seed = seeds[2 * it.x + flip];
     if ((it.w & BATCH_LEAD) && threadIdx.x == 0) seeds[2 * it.x + (flip ^ 1)] = seed + 1u;
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
                                            D.mask2d, D.omega, P, it.y, it.z,
                                            it.w & ~BATCH_LEAD, sr_key(seed));
 }
 
-template <int MODE, int TYP>
+template <int MODE, int TYP, bool NTS>
 hipError_t launch_batch_rows(const BatchLaunch &b)
 {
     using R = typename Store<MODE>::R;
     const auto *doms = (const BatchDom<R> *)b.doms;
     const dim3 grid(b.grid), block((TYP + 3) * 64);
 #define FSLBM_BATCH(PER, DD, F2)                                                        \
-    k_step_batch<MODE, PER, DD, F2, TYP><<<grid, block, 0, b.stream>>>(doms, b.seeds, b.work, \
+    k_step_batch<MODE, PER, DD, F2, TYP, NTS><<<grid, block, 0, b.stream>>>(doms, b.seeds, b.work, \
                                                                         b.nwork, b.flip)
     if (b.periodic) {
         if (b.d1 == 1) { if (b.fd2) FSLBM_BATCH(true, 1, true); else FSLBM_BATCH(true, 1, false); }
@@ -1175,9 +1183,11 @@ hipError_t launch_batch_mode(const BatchLaunch &b)
 {
     const int rows = plan_rows(MODE, b.periodic, b.fd2);
     if constexpr (MODE == FSLBM_F16S)
-        if (rows == F16S_WALLED_ROWS) return launch_batch_rows<MODE, F16S_WALLED_ROWS>(b);
+        if (rows == F16S_WALLED_ROWS) return launch_batch_rows<MODE, F16S_WALLED_ROWS, false>(b);
     if (rows != pipe_rows(MODE)) return hipErrorInvalidValue;
-    return launch_batch_rows<MODE, pipe_rows(MODE)>(b);
+    if constexpr (MODE == FSLBM_F32)
+        if (b.nt) return launch_batch_rows<MODE, pipe_rows(MODE), true>(b);
+    return launch_batch_rows<MODE, pipe_rows(MODE), false>(b);
 }
 
 // Host-side fill of the per-launch parameters of one domain.
@@ -1221,19 +1231,32 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.skey = k.skey;
 }
 
-template <int MODE, bool PERIODIC, int D1, int TYP>
-hipError_t launch_pipe(const void *in, void *out, const TiledAux &aux,
-                       const TiledParams<typename Store<MODE>::R> &P, int grid, hipStream_t s)
+template <int MODE, bool PERIODIC, int D1, int TYP, bool NTS>
+hipError_t launch_pipe_nts(const void *in, void *out, const TiledAux &aux,
+                           const TiledParams<typename Store<MODE>::R> &P, int grid,
+                           hipStream_t s)
 {
     using S = typename Store<MODE>::S;
     using R = typename Store<MODE>::R;
     if (P.c2 != R(0))
-        k_step_pipe<MODE, PERIODIC, D1, true, TYP><<<grid, (TYP + 3) * 64, 0, s>>>(
+        k_step_pipe<MODE, PERIODIC, D1, true, TYP, NTS><<<grid, (TYP + 3) * 64, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     else  // blend = 0: the order-4 arms have zero weight
-        k_step_pipe<MODE, PERIODIC, D1, false, TYP><<<grid, (TYP + 3) * 64, 0, s>>>(
+        k_step_pipe<MODE, PERIODIC, D1, false, TYP, NTS><<<grid, (TYP + 3) * 64, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     return hipGetLastError();
+}
+
+// FP32 steps store non-temporal when the domain's state exceeds the
+// last-level cache (stream_stores); the 16-bit modes never do.
+template <int MODE, bool PERIODIC, int D1, int TYP>
+hipError_t launch_pipe(const Geo &g, const void *in, void *out, const TiledAux &aux,
+                       const TiledParams<typename Store<MODE>::R> &P, int grid, hipStream_t s)
+{
+    if constexpr (MODE == FSLBM_F32)
+        if (stream_stores(2 * state_bytes(g, MODE)))
+            return launch_pipe_nts<MODE, PERIODIC, D1, TYP, true>(in, out, aux, P, grid, s);
+    return launch_pipe_nts<MODE, PERIODIC, D1, TYP, false>(in, out, aux, P, grid, s);
 }
 
 template <int MODE, bool PERIODIC, int D1>
@@ -1252,14 +1275,14 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     else if constexpr (MODE == FSLBM_F16S && !PERIODIC) {
         if (aux.rows == F16S_WALLED_ROWS)
-            return launch_pipe<MODE, PERIODIC, D1, F16S_WALLED_ROWS>(in, out, aux, P, grid, s);
+            return launch_pipe<MODE, PERIODIC, D1, F16S_WALLED_ROWS>(g, in, out, aux, P, grid, s);
     } else if constexpr (MODE == FSLBM_F32 && PERIODIC) {
         if (aux.rows == F32_BOX_ROWS)
-            return launch_pipe<MODE, PERIODIC, D1, F32_BOX_ROWS>(in, out, aux, P, grid, s);
+            return launch_pipe<MODE, PERIODIC, D1, F32_BOX_ROWS>(g, in, out, aux, P, grid, s);
     }
     if constexpr (MODE != FSLBM_F64) {
         if (aux.rows != pipe_rows(MODE)) return hipErrorInvalidValue;
-        return launch_pipe<MODE, PERIODIC, D1, pipe_rows(MODE)>(in, out, aux, P, grid, s);
+        return launch_pipe<MODE, PERIODIC, D1, pipe_rows(MODE)>(g, in, out, aux, P, grid, s);
     }
     return hipGetLastError();
 }
