@@ -10,7 +10,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <queue>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -122,42 +121,54 @@ struct Column {
     int dom, tile, z0, rem;
 };
 
-// Guided static split of the columns of one kernel class into work items.
-// Each cut takes L = (remaining planes of the class) / (2 * CUs) planes,
-// clamped to [lmin, lmax], from the column with the most planes left, so
-// item lengths shrink as the table fills and the last items are short.
-// The items are ordered longest first and dealt to the XCDs in groups (see
-// below), so each XCD gets an even share and the step's tail is one short
-// item.  One item per domain
-// carries BATCH_LEAD.  The cut changes only the schedule: every cell is
-// computed the same way whatever item holds it.
+// Static split of the columns of one kernel class into work items.  One
+// item length L serves the whole table: the one whose items, dealt to the
+// XCDs in rounds of one workgroup per CU, minimise rounds x (L + 3 planes
+// of start-up), the single-domain kernel's step-time model; ties go to the
+// longer items.  Each column is then cut into near-equal chunks of at most
+// L planes.  (Shrinking the length as the table fills, as guided
+// scheduling does, left a single 16-tile pipe of 734 planes with 1024
+// items, 489 of them 8 planes long, against 256 items of 46 planes in one
+// round for the same domain stepped alone.)  The items are ordered longest
+// first and dealt to the XCDs in groups (see below), so each XCD gets an
+// even share.  One item per domain carries BATCH_LEAD.  The cut changes
+// only the schedule: every cell is computed the same way whatever item
+// holds it.
 std::vector<int4> guided_split(std::vector<Column> &cols, int lmax, int cus, size_t ndom)
 {
-    const int lmin = std::min(8, lmax);
-    long long left = 0;
-    for (const Column &c : cols) left += c.rem;
-    auto fewer = [&](int a, int b) {
-        return cols[a].rem < cols[b].rem || (cols[a].rem == cols[b].rem && a > b);
-    };
-    std::priority_queue<int, std::vector<int>, decltype(fewer)> pq(fewer);
-    for (int k = 0; k < (int)cols.size(); k++)
-        if (cols[k].rem > 0) pq.push(k);
-    std::vector<std::pair<int, int4>> items;
-    while (!pq.empty()) {
-        const int k = pq.top();
-        pq.pop();
-        Column &c = cols[k];
-        long long L = (left + 2LL * cus - 1) / (2LL * cus);
-        L = std::max<long long>(lmin, std::min<long long>(lmax, L));
-        int take = (int)std::min<long long>(L, c.rem);
-        if (c.rem - take < lmin)  // no sliver at the column end
-            take = c.rem <= lmax ? c.rem : c.rem / 2;
-        items.push_back({take, make_int4(c.dom, c.tile, c.z0, c.z0 + take)});
-        c.z0 += take;
-        c.rem -= take;
-        left -= take;
-        if (c.rem > 0) pq.push(k);
+    const int lmin = std::min(4, lmax);
+    int longest = 0;
+    for (const Column &c : cols) longest = std::max(longest, c.rem);
+    const long long slots = std::max(1, cus / 8);
+    int L = std::max(1, std::min(lmax, longest));
+    double best = -1.0;
+    for (int len = std::min(lmax, std::max(longest, 1)); len >= lmin; len--) {
+        long long items = 0;
+        for (const Column &c : cols) items += (c.rem + len - 1) / len;
+        const long long rounds = ((items + 7) / 8 + slots - 1) / slots;
+        const double cost = (double)rounds * (len + 3);
+        if (best < 0 || cost < best) {
+            best = cost;
+            L = len;
+        }
     }
+    // chunk-major across the columns (a domain's tiles are consecutive
+    // columns), so equal-length items of neighbouring tiles and one z range
+    // stay next to each other through the sort below
+    std::vector<std::pair<int, int4>> items;
+    int maxch = 0;
+    for (const Column &c : cols)
+        if (c.rem > 0) maxch = std::max(maxch, (c.rem + L - 1) / L);
+    for (int k = 0; k < maxch; k++)
+        for (const Column &c : cols) {
+            if (c.rem <= 0) continue;
+            const int nch = (c.rem + L - 1) / L;
+            if (k >= nch) continue;
+            const int q = c.rem / nch, r = c.rem - q * nch;
+            const int z = c.z0 + k * q + std::min(k, r);
+            const int take = q + (k < r ? 1 : 0);
+            items.push_back({take, make_int4(c.dom, c.tile, z, z + take)});
+        }
     std::stable_sort(items.begin(), items.end(),
                      [](const auto &x, const auto &y) { return x.first > y.first; });
     // Deal groups of G consecutive items (neighbouring tiles of one z
