@@ -134,7 +134,7 @@ struct Column {
 // even share.  One item per domain carries BATCH_LEAD.  The cut changes
 // only the schedule: every cell is computed the same way whatever item
 // holds it.
-std::vector<int4> guided_split(std::vector<Column> &cols, int lmax, int cus, size_t ndom)
+std::vector<int4> split_columns(std::vector<Column> &cols, int lmax, int cus, size_t ndom)
 {
     const int lmin = std::min(4, lmax);
     int longest = 0;
@@ -230,7 +230,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     b->nt = stream_stores(stepped);
     const int cus = device_cus();
     std::vector<int4> work[NCLASS];
-    for (int c = 0; c < NCLASS; c++) work[c] = guided_split(cols[c], ch > 0 ? ch : 128, cus, nd);
+    for (int c = 0; c < NCLASS; c++) work[c] = split_columns(cols[c], ch > 0 ? ch : 128, cus, nd);
     b->flip = 0;
     if (host.size() > b->d_doms_cap) {
         if (b->d_doms) (void)hipFree(b->d_doms);
