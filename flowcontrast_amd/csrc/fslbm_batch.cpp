@@ -110,8 +110,12 @@ StepArgs step_args(const Domain &d)
     return a;
 }
 
+// Destroy the captured graphs, after their launches on the batch's stream
+// have finished.
 void drop_graphs(fslbm_batch *b)
 {
+    if (!b->gexec[0] && !b->gexec[1]) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto &g : b->gexec)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
@@ -232,6 +236,11 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     std::vector<int4> work[NCLASS];
     for (int c = 0; c < NCLASS; c++) work[c] = split_columns(cols[c], ch > 0 ? ch : 128, cus, nd);
     b->flip = 0;
+    // The captured graphs hold the current tables' addresses: let their
+    // launches finish and destroy them before any table is freed or
+    // rewritten, so that no graph exec outlives memory it refers to.
+    BCHK(b, hipStreamSynchronize(b->stream));
+    drop_graphs(b);
     if (host.size() > b->d_doms_cap) {
         if (b->d_doms) (void)hipFree(b->d_doms);
         b->d_doms = nullptr;
@@ -264,7 +273,6 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         b->nwork[c] = (int)work[c].size();
     }
     BCHK(b, hipStreamSynchronize(b->stream));
-    drop_graphs(b);
     b->dirty = false;
     return FSLBM_OK;
 }
@@ -293,23 +301,35 @@ hipError_t launch_step_all(fslbm_batch *b, int flip)
     return hipSuccess;
 }
 
+// Capture the two graphs (two steps each, from flip 0 and from flip 1).
+// Either both are made or neither: a failure destroys what was made, so
+// that no launch ever finds one graph of the pair missing.
 int capture_graphs(fslbm_batch *b)
 {
+    drop_graphs(b);
     for (int start = 0; start < 2; start++) {
         hipGraph_t g = nullptr;
-        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e0 = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        if (e0 != hipSuccess) {
+            drop_graphs(b);
+            return bfail(b, FSLBM_EHIP, "graph capture failed: %s", hipGetErrorString(e0));
+        }
         hipError_t e1 = launch_step_all(b, start);
         hipError_t e2 = launch_step_all(b, start ^ 1);
         hipError_t e3 = hipStreamEndCapture(b->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
+            drop_graphs(b);
             return bfail(b, FSLBM_EHIP, "graph capture failed: %s",
                          hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
         }
         hipError_t e4 = hipGraphInstantiate(&b->gexec[start], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (e4 != hipSuccess)
+        if (e4 != hipSuccess) {
+            b->gexec[start] = nullptr;
+            drop_graphs(b);
             return bfail(b, FSLBM_EHIP, "graph instantiate failed: %s", hipGetErrorString(e4));
+        }
     }
     return FSLBM_OK;
 }
@@ -324,7 +344,7 @@ int step_impl(fslbm_batch *b, int nsteps)
     for (int c = 0; c < NCLASS; c++) any |= b->nwork[c] > 0;
     int s = 0;
     if (any && b->use_graph && nsteps >= 2) {
-        if (!b->gexec[0]) {
+        if (!b->gexec[0] || !b->gexec[1]) {
             const int rc = capture_graphs(b);
             if (rc) return rc;
         }
@@ -701,7 +721,7 @@ int fslbm_batch_time_steps(fslbm_batch_t b, int nsteps, double *ms)
         const int rc = rebuild(b);
         if (rc) return rc;
     }
-    if (b->use_graph && !b->gexec[0]) {
+    if (b->use_graph && (!b->gexec[0] || !b->gexec[1])) {
         const int rc = capture_graphs(b);
         if (rc) return rc;
     }

@@ -124,6 +124,7 @@ std::vector<uint8_t> read_mask(const Opts &o)
         int v = 0;
         const auto r = std::from_chars(p, end, v);
         if (r.ec != std::errc()) die("pipe.txt: not an integer");
+        if (v != 0 && v != 1 && v != 125) die("pipe.txt: mask codes must be 0, 1 or 125");
         p = r.ptr;
         m[i] = (uint8_t)v;
     }

@@ -3,7 +3,10 @@ reference's main(), FSLBM.cu:489-909): it reads pipe.txt / param.txt as
 pipe.m writes them, and writes qp.txt, the averaged YZ-plane statistics,
 the instantaneous planes, IO.txt and the restart backups in the
 reference's formats, on the reference's cadences.  The numbers it writes
-are checked against the same schedule run through the library."""
+are checked against the same schedule run through the library, and against
+the CPU oracle's run of the reference schedule in the reference's padded
+layout (monitoring, statistics, planes, restart volumes and both restart
+paths)."""
 import subprocess
 from pathlib import Path
 
@@ -136,3 +139,151 @@ def test_cli_exact_checkpoint_resumes_bitwise(tmp_path, precision):
     for name in ("qp.txt", "_p.txt", "_uz.txt", "_uz2.txt", "_uxz2.txt", "_av.txt", "p.txt",
                  "uz.txt", "uz1.txt"):
         assert (a / name).read_text() == (b / name).read_text(), name
+
+
+# ----------------------------------------------------------------------
+# the files against the CPU oracle in the reference's padded layout
+# ----------------------------------------------------------------------
+def param_values():
+    """param.txt as the CLI reads it: %e into float (FSLBM.cu:502-507)."""
+    tb, pin, nu, d1, delta, blend = PARAMS
+    return (float(np.float32(tb)), float(np.float32(pin)), float(np.float32(nu)), d1, delta,
+            float(np.float32(blend)))
+
+
+def oracle_schedule(orc, padded, steps, monitor, stats, dtype, state=None, acc=None, c0=0,
+                    p2=0.0):
+    """The reference's main loop (FSLBM.cu:620-764) on the padded layout:
+    FSLBMcalc + bcond with the pending p2, the inlet/outlet monitoring every
+    `monitor` steps (setting p2), and every `stats` steps the YZ-plane
+    statistics at x = lx/2.  Returns the qp rows, the statistics
+    accumulator, the state, the planes of the last statistics step and the
+    pending p2.  c0 and p2 continue an earlier run (its step count and the
+    gauge its last monitoring step left for the next step)."""
+    tb, pin, nu, d1, delta, blend = param_values()
+    op = orc.params(tb, pin, nu, blend, d1)
+    lz, ly, lx = padded.shape
+    if state is None:  # initphys: p = pin (lz - d2 - z) / (lz - 2 d2), u = 0
+        z = np.arange(lz)[:, None, None]
+        p = np.where(padded != 125, pin * (lz - FRAME - z) / (lz - 2 * FRAME), 0.0)
+        state = [p.astype(dtype)] + [np.zeros(padded.shape, dtype) for _ in range(3)]
+    if acc is None:
+        acc = orc.Stats(ly * lz)
+    l1 = float((padded[FRAME] == 0).sum())
+    r = (lx - 2 * FRAME) // 2
+    qp, planes = [], None
+    for c in range(c0 + 1, c0 + steps + 1):
+        state = orc.step_padded(state, op, padded, FRAME, p2=p2, dtype=dtype)
+        p2 = 0.0
+        if c % monitor:
+            continue
+        q1, q2, p1, p2 = orc.monitor_padded(state, FRAME, l1)
+        ff = (p1 - p2) / q1 / q1 / 3.0 * 4.0 * r / (lz - 2 * FRAME)
+        qp.append([q1, q2, p1, p2, ff, float(state[3][lz // 2, ly // 2, lx // 2])])
+        if c % stats == 0:
+            yz = [np.asarray(f, np.float64)[:, :, lx // 2].reshape(-1) for f in state]
+            acc.accumulate(yz, delta)
+            planes = yz + [np.asarray(f, np.float64)[lz // 2].reshape(-1) for f in state]
+    return np.array(qp), acc, state, planes, p2
+
+
+def check_files(d: Path, qp_ref, acc, planes, rtol, atol_rel, rows=None):
+    qp = np.loadtxt(d / "qp.txt", ndmin=2)
+    if rows is not None:
+        qp = qp[rows]
+    assert qp.shape == qp_ref.shape
+    for k in range(6):
+        # q1, q2 share the velocity scale and p1, p2 the pressure scale (p2
+        # is zero up to roundoff once the gauge has been applied)
+        cols = {0: [0, 1], 1: [0, 1], 2: [2, 3], 3: [2, 3]}.get(k, [k])
+        scale = np.abs(qp_ref[:, cols]).max()
+        np.testing.assert_allclose(qp[:, k], qp_ref[:, k], rtol=rtol, atol=atol_rel * scale,
+                                   err_msg=f"qp.txt column {k}")
+    a, cnt, cnt2 = acc.read()
+    assert (d / "_av.txt").read_text().split() == [str(cnt), str(cnt2)]
+    names = ["_p", "_ux", "_uy", "_uz", "_p2", "_ux2", "_uy2", "_uz2", "_uxy2", "_uxz2", "_uyz2"]
+    exp = [a[k] / (cnt if k < 4 else cnt2) for k in range(11)]
+    # Scales: the velocity components share one (in this axisymmetric flow
+    # ux and uy are roundoff-sized); a deviation takes its mean field's
+    # scale (the pressure is stationary here, so |dev p| is roundoff), and
+    # a product of deviations the velocity scale squared.
+    ps = np.abs(exp[0]).max()
+    us = max(np.abs(exp[j]).max() for j in (1, 2, 3, 5, 6, 7))
+    scales = [ps, us, us, us, ps, us, us, us, us * us, us * us, us * us]
+    for k, name in enumerate(names):
+        scale = scales[k]
+        np.testing.assert_allclose(read_col(d / f"{name}.txt"), exp[k], rtol=rtol,
+                                   atol=atol_rel * scale + 1e-30, err_msg=name)
+    files = ["p.txt", "ux.txt", "uy.txt", "uz.txt", "p1.txt", "ux1.txt", "uy1.txt", "uz1.txt"]
+    for k, (name, e) in enumerate(zip(files, planes)):
+        js = [k] if k % 4 == 0 else [4 * (k // 4) + j for j in (1, 2, 3)]
+        scale = max(np.abs(planes[j]).max() for j in js)
+        np.testing.assert_allclose(read_col(d / name), e, rtol=rtol,
+                                   atol=atol_rel * scale + 1e-30, err_msg=name)
+
+
+# FP64: the files hold %e text (7 significant digits) of values that agree
+# with the oracle to ~1e-12; FP32: relative 1e-5 of each field's scale.
+ORACLE_TOL = {"f64": (2e-6, 1e-9), "f32": (1e-5, 1e-5)}
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_cli_files_match_padded_oracle(tmp_path, orc, precision):
+    """fslbm_run's qp.txt, averaged statistics, instantaneous YZ/XY planes
+    and restart volumes against the oracle's padded-layout run of the
+    reference schedule; then both restart paths continue it: the exact
+    checkpoint and the reference's float32 volumes with %e averages."""
+    rtol, atol = ORACLE_TOL[precision]
+    dtype = np.float64 if precision == "f64" else np.float32
+    padded = write_inputs(tmp_path)
+    run_cli(tmp_path, "--steps", 100, "--monitor", 50, "--stats", 50, "--backup", 100,
+            "--precision", precision, "--fresh")
+    qp_ref, acc, state, planes, p2 = oracle_schedule(orc, padded, 100, 50, 50, dtype)
+    check_files(tmp_path, qp_ref, acc, planes, rtol, atol)
+    lz, ly, lx = padded.shape
+    uscale = max(np.abs(np.asarray(state[f], np.float64)).max() for f in (1, 2, 3))
+    for f, v in enumerate(("p3Dii", "ux3Dii", "uy3Dii", "uz3Dii")):
+        vol = np.fromfile(tmp_path / v, dtype=np.float32).reshape(lz, ly, lx)
+        ref = np.asarray(state[f], np.float64)
+        scale = np.abs(ref).max() if f == 0 else uscale
+        np.testing.assert_allclose(vol, ref.astype(np.float32), rtol=max(rtol, 1e-6),
+                                   atol=max(atol, 1e-7) * scale + 1e-30, err_msg=v)
+
+    # (1) exact checkpoint: 50 more steps continue the oracle's run
+    exact = tmp_path / "exact"
+    exact.mkdir()
+    for name in tmp_path.iterdir():
+        if name.is_file():
+            (exact / name.name).write_bytes(name.read_bytes())
+    run_cli(exact, "--steps", 50, "--monitor", 50, "--stats", 50, "--backup", 0,
+            "--precision", precision)
+    acc1 = orc.Stats(ly * lz)
+    a0, c0, c20 = acc.read()
+    acc1.acc[:] = a0.reshape(-1)
+    acc1.cnt.value, acc1.cnt2.value = c0, c20
+    qp1, acc1, _, planes1, _ = oracle_schedule(orc, padded, 50, 50, 50, dtype, state=state,
+                                               acc=acc1, c0=100, p2=p2)
+    check_files(exact, np.vstack([qp_ref, qp1]), acc1, planes1, rtol, atol)
+
+    # (2) the reference's restart files: float32 volumes and %e averages
+    # (FSLBM.cu:386-468); the oracle restarts from the same float32 values
+    vols = tmp_path / "vols"
+    vols.mkdir()
+    for name in tmp_path.iterdir():
+        if name.is_file() and name.name != "fslbm_ckpt.bin":
+            (vols / name.name).write_bytes(name.read_bytes())
+    run_cli(vols, "--steps", 50, "--monitor", 50, "--stats", 50, "--backup", 0,
+            "--precision", precision)
+    st32 = [np.fromfile(vols / v, dtype=np.float32).reshape(lz, ly, lx).astype(dtype)
+            for v in ("p3Dii", "ux3Dii", "uy3Dii", "uz3Dii")]
+    acc2 = orc.Stats(ly * lz)
+    cnt, cnt2 = map(int, (vols / "_avii.txt").read_text().split())
+    names = ["_p", "_ux", "_uy", "_uz", "_p2", "_ux2", "_uy2", "_uz2", "_uxy2", "_uxz2", "_uyz2"]
+    acc2.acc[:] = np.concatenate([
+        read_col(vols / f"{n}ii.txt").astype(np.float32).astype(np.float64)
+        * (cnt if k < 4 else cnt2) for k, n in enumerate(names)])
+    acc2.cnt.value, acc2.cnt2.value = cnt, cnt2
+    # the reference restarts with no pending gauge (p2 is not saved)
+    qp2, acc2, _, planes2, _ = oracle_schedule(orc, padded, 50, 50, 50, dtype, state=st32,
+                                               acc=acc2, c0=100)
+    check_files(vols, qp2, acc2, planes2, max(rtol, 1e-5), max(atol, 1e-6), rows=[2])

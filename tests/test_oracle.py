@@ -43,6 +43,44 @@ def test_rest_state_with_walls_is_fixed(orc):
         assert np.abs(out[f]).max() < 1e-16
 
 
+@pytest.mark.parametrize("geom", ["rough", "orifice", "sphere"])
+def test_rest_state_is_fixed_for_z_varying_masks(orc, geom):
+    """Every fluid cell at rest keeps its pressure whatever its wall
+    neighbours are (the wall rule substitutes the own pressure, FSLBM.cu:
+    137-143).  A neighbour's wall test read at the wrong z (or x, y) would
+    take a wall's stored zero pressure instead and break this."""
+    from test_gpu_geometry import GEOMETRIES
+    mask = GEOMETRIES[geom]()
+    assert any(not np.array_equal(mask[0], mask[k]) for k in range(1, mask.shape[0]))
+    p = np.where(mask == 0, 0.37, 0.0)
+    z = np.zeros_like(p)
+    for b, d1 in ((0.0, 2), (1.0, 2), (0.5, 1)):
+        prm = orc.params(tblend=0.8, nu=0.02, blend=b, d1=d1)
+        out = orc.step_folded([p, z, z, z], prm, mask=mask)
+        assert np.abs(out[0] - p).max() < 1e-15
+        for f in (1, 2, 3):
+            assert np.abs(out[f]).max() < 1e-16
+
+
+def test_z_mirror_symmetry_with_z_varying_mask(orc):
+    """Mirroring z (and the axial velocity's sign) commutes with the step
+    when there is no jump: a z-asymmetric obstacle and a random state."""
+    from test_gpu_geometry import sphere_duct
+    mask = sphere_duct()
+    rng = np.random.default_rng(3)
+    fl = mask == 0
+    st = [np.where(fl, 0.3 + 0.01 * rng.standard_normal(mask.shape), 0.0)] + \
+         [np.where(fl, 0.02 * rng.standard_normal(mask.shape), 0.0) for _ in range(3)]
+    prm = orc.params(tblend=0.9, nu=0.02, blend=0.5, d1=2)
+    out = orc.step_folded(st, prm, mask=mask, nsteps=3)
+    mir = [a[::-1].copy() for a in st]
+    mir[3] = -mir[3]
+    outm = orc.step_folded(mir, prm, mask=mask[::-1].copy(), nsteps=3)
+    for f in range(4):
+        back = outm[f][::-1] * (-1 if f == 3 else 1)
+        assert np.abs(back - out[f]).max() < 1e-15
+
+
 def test_uniform_flow_is_fixed_periodic(orc):
     n = 10
     U = (0.031, -0.017, 0.024)
