@@ -40,10 +40,11 @@ EXPORTS = [
     "fslbm_server_last_error",
     "fslbm_server_clock", "fslbm_server_submit", "fslbm_server_pump",
     "fslbm_server_next_done", "fslbm_server_fetch", "fslbm_server_drop",
-    "fslbm_server_query",
+    "fslbm_server_query", "fslbm_step_noalloc",
 ]
 EBUSY = -6
 SERVE_SHARE, SERVE_SHORTEST = 0, 1  # serving policies
+NOALLOC_MAX_COLUMNS = 1 << 19  # FSLBM_NOALLOC_MAX_COLUMNS
 
 
 class Params(C.Structure):
@@ -175,6 +176,7 @@ def lib() -> C.CDLL:
         "fslbm_server_drop": (i, [vp, C.c_longlong]),
         "fslbm_server_query": (i, [vp, ip, ip, ip, C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]),
+        "fslbm_step_noalloc": (i, [C.POINTER(Params), vp, i, i, i, i, vp, vp, d, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -45,6 +45,7 @@ LIB_SOURCES = [
     "fslbm_serve.cpp",
     "fslbm_api.cpp",
     "fslbm_inputs.cpp",
+    "fslbm_noalloc.hip",
 ]
 
 

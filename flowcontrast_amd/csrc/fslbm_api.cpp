@@ -86,6 +86,14 @@ int fail(fslbm_ctx *c, int code, const char *fmt, ...)
 
 thread_local std::string g_create_err;
 
+}  // namespace
+
+namespace fslbm {
+void set_thread_error(const char *msg) { g_create_err = msg; }
+}  // namespace fslbm
+
+namespace {
+
 int check_params(fslbm_ctx *c, const fslbm_params *p)
 {
     if (!p) return fail(c, FSLBM_EINVAL, "params is NULL");

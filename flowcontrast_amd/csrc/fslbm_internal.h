@@ -185,6 +185,10 @@ hipError_t launch_stats(const Geo &g, int precision, const void *state,
 hipError_t launch_moments(const Geo &g, int precision, const void *state,
                           const uint8_t *mask, const Quant &q, double *acc,
                           hipStream_t s);
+// Message of the last failed call without a handle (fslbm_create,
+// fslbm_step_noalloc), read by fslbm_last_error(NULL); per host thread.
+void set_thread_error(const char *msg);
+
 // Fill n doubles with zero (kernel, usable under graph capture).
 hipError_t launch_zero(double *dst, long long n, hipStream_t s);
 

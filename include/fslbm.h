@@ -396,6 +396,31 @@ int fslbm_server_drop(fslbm_server_t s, long long id);
 int fslbm_server_query(fslbm_server_t s, int *queued, int *running,
                        int *finished, size_t *used, size_t *budget);
 
+/* ------------------------------------------------------------------ */
+/* One step of caller-owned device state, allocating nothing.  No        */
+/* counterpart in the reference; for callers that cannot allocate       */
+/* device memory (code under HIP graph capture, tight serving loops).   */
+/* It is fslbm_step without a context: in and out are device buffers of */
+/* four SoA planes of nx*ny*nz values (p, ux, uy, uz; double for        */
+/* FSLBM_F64, float for FSLBM_F32), in != out; mask2d is one z plane of */
+/* nx*ny codes 0/125 on the device (the mask of every plane, with a     */
+/* wall frame on the x/y faces), or NULL for a fully periodic box.  The */
+/* wall cells of `in` must hold 0 in every field (the state after a     */
+/* bcond, FSLBM.cu:290); every cell of `out` is written (walls with 0). */
+/* gauge is subtracted from the new pressure (bcond's p2, FSLBM.cu:628).*/
+/* The step is queued on `stream` (a hipStream_t; NULL: the default    */
+/* stream) and the call returns without waiting.  d1 must be 1 or 2 and */
+/* nz >= 16.  The per-column wall weights the kernel needs are built on */
+/* the device into a fixed table of the library (FSLBM_NOALLOC_MAX_     */
+/* COLUMNS columns; a walled box with more columns is FSLBM_EINVAL),    */
+/* one per GPU, which every call on that GPU rewrites: calls that       */
+/* overlap on one GPU race.  Errors: fslbm_last_error(NULL).            */
+/* ------------------------------------------------------------------ */
+#define FSLBM_NOALLOC_MAX_COLUMNS (1 << 19)
+int fslbm_step_noalloc(const fslbm_params *prm, const uint8_t *mask2d, int nx, int ny,
+                       int nz, int precision, const void *in, void *out, double gauge,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif
