@@ -380,8 +380,9 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
         return FSLBM_ENOMEM;
     }
     if (precision == FSLBM_F16S || precision == FSLBM_BF16S) {
-        // Zero state: reference 0, unit scales until a state is uploaded.
-        c->q = Quant{1.0f / 1024, 1024.0f, 1.0f / 1024, 1024.0f, 0.0f,
+        // Zero state: reference 0 and the smallest pressure scale (as
+        // choose_quant gives for a quiet start) until a state is uploaded.
+        c->q = Quant{1.0f / 32768, 32768.0f, 1.0f, 1.0f, 0.0f,
                      c->g.periodic ? 0.0f : (float)(-prm->pin / nz)};
     }
     *out = c;
@@ -553,7 +554,8 @@ struct SnapHeader {
     uint32_t pad2;
     double gauge;     // pending gauge of the next step
 };
-constexpr char SNAP_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'S', 'N', '1'};
+// Version 2: velocities of the 16-bit modes stored unscaled (su = 1).
+constexpr char SNAP_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'S', 'N', '2'};
 
 }  // namespace
 
@@ -591,7 +593,8 @@ int fslbm_snapshot_load(fslbm_t c, const void *buf, long long bytes)
     SnapHeader h;
     if (bytes < (long long)sizeof h) return fail(c, FSLBM_EINVAL, "snapshot too short");
     std::memcpy(&h, buf, sizeof h);
-    if (std::memcmp(h.magic, SNAP_MAGIC, 8) != 0) return fail(c, FSLBM_EINVAL, "not a snapshot");
+    if (std::memcmp(h.magic, SNAP_MAGIC, 8) != 0)
+        return fail(c, FSLBM_EINVAL, "not a snapshot of this storage format");
     const long long need = fslbm_snapshot_size(c);
     if (h.precision != c->precision || h.n != c->g.N || bytes < need)
         return fail(c, FSLBM_EINVAL, "snapshot of precision %d and %lld cells (%lld bytes) for "
@@ -714,10 +717,11 @@ int fslbm_probe(fslbm_t c, int x, int y, int z, double out[4])
     const size_t sb = storage_bytes(c->precision);
     const long long i = x + (long long)c->g.nx * (y + (long long)c->g.ny * z);
     unsigned char raw[4][8];
-    for (int f = 0; f < 4; f++)
-        HIPCHK(c, hipMemcpyAsync(raw[f], (const char *)c->state[c->cur] +
-                                             sb * (f * c->g.N + i),
-                                 sb, hipMemcpyDeviceToHost, c->stream));
+    for (int f = 0; f < 4; f++) {
+        const long long e = state_at(c->precision, f, i, c->g.N);
+        HIPCHK(c, hipMemcpyAsync(raw[f], (const char *)c->state[c->cur] + sb * e, sb,
+                                 hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const bool wall = !c->g.periodic && c->hmask[i] != 0;
     auto dec = [&](auto tag, int f) -> double {

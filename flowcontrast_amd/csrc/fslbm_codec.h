@@ -71,10 +71,9 @@ inline float pow2_at_least(double v)
     return std::ldexp(1.0f, e);
 }
 
-// Shifted-storage reference and scales from the state: the pressure
-// reference is the mean plus the mean axial gradient -pin/nz (walled
-// domains); scales map the largest magnitude to [512, 1024) and are
-// powers of two, so scaling is exact.
+// Shifted-storage reference and pressure scale from the state: the
+// pressure reference is the mean plus the mean axial gradient -pin/nz
+// (walled domains); the scale is a power of two, so scaling is exact.
 template <typename T>
 void choose_quant(const Geo &g, const HostMask &hm, double pin, const T *const src[4],
                   Quant &q)
@@ -103,35 +102,29 @@ void choose_quant(const Geo &g, const HostMask &hm, double pin, const T *const s
         n += bcnt[b];
     }
     const double base = n ? sum / (double)n : 0.0;
-    std::vector<double> bp(nb, 0.0), bu(nb, 0.0);
+    std::vector<double> bp(nb, 0.0);
     par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long b) {
-        double pmax = 0, umax = 0;
+        double pmax = 0;
         for (long long i = lo; i < hi; i++) {
             if (hm.wall(i)) continue;
             const double dp = (double)src[0][i] - (base + slope * (double)(i / nxy));
             pmax = std::max(pmax, std::fabs(dp));
-            for (int f = 1; f < 4; f++) umax = std::max(umax, std::fabs((double)src[f][i]));
         }
         bp[b] = pmax;
-        bu[b] = umax;
     });
-    double pmax = 0, umax = 0;
-    for (long long b = 0; b < nb; b++) {
-        pmax = std::max(pmax, bp[b]);
-        umax = std::max(umax, bu[b]);
-    }
-    // Floors leave room for growth from a quiet start (the fp16 range
-    // reaches 64x past the scaled maximum of 1024, and small values keep
-    // their 11 significant bits): 1e-3 for velocity; for pressure the
-    // acoustic bound of a velocity perturbation, |dp| <= sqrt(3)|du| <
-    // 2 umax, and a tenth of the jump.  The jump itself is carried by the
-    // reference line pbase + pslope*z.
-    umax = std::max(umax, 1e-3);
-    pmax = std::max(pmax, std::max(2.0 * umax, 0.1 * std::fabs(pin)));
-    const float su = pow2_at_least(umax) / 1024.0f;
-    const float sp = pow2_at_least(pmax) / 1024.0f;
-    q.su = su;
-    q.inv_su = 1.0f / su;
+    double pmax = 0;
+    for (long long b = 0; b < nb; b++) pmax = std::max(pmax, bp[b]);
+    // Velocities are stored unscaled (su = 1, see Store<FSLBM_F16S>).  The
+    // pressure deviation is scaled by a power of two sp: the largest
+    // initial deviation maps into [512, 1024), 64x below the binary16
+    // maximum, and sp >= 2^-15 keeps deviations up to 2 representable
+    // whatever the initial state (a quiet start has none): past any
+    // pressure a stable lattice flow reaches, |dp| <~ sqrt(3)|u| with |u|
+    // below the lattice speed of sound.  Deviations above 2^-14 sp keep 11
+    // significant bits, so the floor costs no precision that matters.
+    const float sp = std::max(pow2_at_least(pmax) / 1024.0f, std::ldexp(1.0f, -15));
+    q.su = 1.0f;
+    q.inv_su = 1.0f;
     q.sp = sp;
     q.inv_sp = 1.0f / sp;
     q.pbase = (float)base;
@@ -153,9 +146,10 @@ void encode_state(const Geo &g, const HostMask &hm, int precision, const Quant &
             for (long long i = lo; i < hi; i++) {
                 const bool wall = hm.wall(i);
                 const int z = (int)(i / nxy);
-                d[i] = wall ? S(0) : ST::st_p((R)src[0][i], z, q);
+                using L = StateLayout<ST::mode>;
+                d[L::at(0, i, N)] = wall ? S(0) : ST::st_p((R)src[0][i], z, q);
                 for (int f = 1; f < 4; f++)
-                    d[f * N + i] = wall ? S(0) : ST::st_u((R)src[f][i], q);
+                    d[L::at(f, i, N)] = wall ? S(0) : ST::st_u((R)src[f][i], q);
             }
         });
     };
@@ -181,9 +175,10 @@ void decode_state(const Geo &g, const HostMask &hm, int precision, const Quant &
             for (long long i = lo; i < hi; i++) {
                 const bool wall = hm.wall(i);
                 const int z = (int)(i / nxy);
-                dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[i], z, q);
+                using L = StateLayout<ST::mode>;
+                dst[0][i] = wall ? T(0) : (T)ST::ld_p(s[L::at(0, i, N)], z, q);
                 for (int f = 1; f < 4; f++)
-                    dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[f * N + i], q);
+                    dst[f][i] = wall ? T(0) : (T)ST::ld_u(s[L::at(f, i, N)], q);
             }
         });
     };

@@ -19,8 +19,9 @@ struct Geo {
 };
 
 // Dequantisation of the 16-bit shifted storage modes:
-//   p = pbase + pslope*z + sp*h_p,   u = su*h_u.
-// Powers of two for sp/su keep the scaling exact.
+//   p = pbase + pslope*z + sp*h_p,   u = h_u.
+// sp is a power of two (exact scaling); su is 1 (velocities are stored
+// unscaled) and kept so that snapshots carry the whole record.
 struct Quant {
     float sp, inv_sp, su, inv_su, pbase, pslope;
 };

@@ -97,10 +97,12 @@ __global__ __launch_bounds__(256) void k_step_plain(
     const int i = x + nx * y + nxy * z;
     if (!PERIODIC && mask[i] != 0) return;  // walls stay 0 in both buffers
 
-    const auto *P = in, *U = in + N, *V = in + 2 * N, *W = in + 3 * N;
-    const R ps = ST::ld_p(P[i], z, q);
-    const R us = ST::ld_u(U[i], q), vs = ST::ld_u(V[i], q),
-            ws = ST::ld_u(W[i], q);
+    using L = StateLayout<MODE>;
+    // field f of cell j (32-bit offsets: 4*N < 2^31)
+    auto F = [&](int f, int j) { return in[(int)L::at(f, j, N)]; };
+    const R ps = ST::ld_p(F(0, i), z, q);
+    const R us = ST::ld_u(F(1, i), q), vs = ST::ld_u(F(2, i), q),
+            ws = ST::ld_u(F(3, i), q);
 
     R rt = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll
@@ -121,9 +123,9 @@ __global__ __launch_bounds__(256) void k_step_plain(
                 if (PERIODIC) xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
                 const int j = xx + nx * yy + nxy * zz;
                 const bool wall = !PERIODIC && mask[j] != 0;
-                const R pn = wall ? ps : ST::ld_p(P[j], zz, q) + jump;
-                const R un = ST::ld_u(U[j], q), vn = ST::ld_u(V[j], q),
-                        wn = ST::ld_u(W[j], q);
+                const R pn = wall ? ps : ST::ld_p(F(0, j), zz, q) + jump;
+                const R un = ST::ld_u(F(1, j), q), vn = ST::ld_u(F(2, j), q),
+                        wn = ST::ld_u(F(3, j), q);
                 const R e_u = (R)cx * un + (R)cy * vn + (R)cz * wn;
                 const R t = (R)3 * e_u;
                 const R uu = (R)3 * (un * un + vn * vn + wn * wn);
@@ -162,9 +164,9 @@ __global__ __launch_bounds__(256) void k_step_plain(
         R sx = 0, sy = 0, sz = 0;
 #pragma unroll
         for (int t = 0; t < 6; t++) {
-            sx += ST::ld_u(U[o[t]], q);
-            sy += ST::ld_u(V[o[t]], q);
-            sz += ST::ld_u(W[o[t]], q);
+            sx += ST::ld_u(F(1, o[t]), q);
+            sy += ST::ld_u(F(2, o[t]), q);
+            sz += ST::ld_u(F(3, o[t]), q);
         }
         Lx += cr * sx;
         Ly += cr * sy;
@@ -172,12 +174,11 @@ __global__ __launch_bounds__(256) void k_step_plain(
     }
 
     const R g = k.dtddy * k.m;
-    auto *Po = out, *Uo = out + N, *Vo = out + 2 * N, *Wo = out + 3 * N;
     const SrWords rw = ST::SR ? sr_words((uint32_t)i, k.skey) : SrWords{0u, 0u};
-    Po[i] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
-    Uo[i] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
-    Vo[i] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);
-    Wo[i] = ST::st_u_sr((az + g * Lz) * k.c + ws * k.cc, q, rw.b << 16);
+    out[L::at(0, i, N)] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
+    out[L::at(1, i, N)] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
+    out[L::at(2, i, N)] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);
+    out[L::at(3, i, N)] = ST::st_u_sr((az + g * Lz) * k.c + ws * k.cc, q, rw.b << 16);
 }
 
 template <int MODE>
@@ -267,10 +268,11 @@ __global__ __launch_bounds__(256) void k_reduce(
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
         if (mask && mask[i] != 0) continue;
-        const double p = ST::ld_p(st[i], i / nxy, q);
-        const double u = ST::ld_u(st[N + i], q);
-        const double w = ST::ld_u(st[2 * N + i], q);
-        const double z = ST::ld_u(st[3 * N + i], q);
+        using L = StateLayout<MODE>;
+        const double p = ST::ld_p(st[L::at(0, i, N)], i / nxy, q);
+        const double u = ST::ld_u(st[L::at(1, i, N)], q);
+        const double w = ST::ld_u(st[L::at(2, i, N)], q);
+        const double z = ST::ld_u(st[L::at(3, i, N)], q);
         v[0] += 1.0;
         v[1] += p;
         v[2] += u;
@@ -324,8 +326,8 @@ __global__ __launch_bounds__(256) void k_monitor(
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nxy; i += gridDim.x * 256) {
         if (mask && mask[i] != 0) continue;
         v[0] += 1.0;
-        v[1] += ST::ld_p(st[i], 0, q);
-        v[2] += ST::ld_u(st[3 * N + i], q);
+        v[1] += ST::ld_p(st[StateLayout<MODE>::at(0, i, N)], 0, q);
+        v[2] += ST::ld_u(st[StateLayout<MODE>::at(3, i, N)], q);
     }
     block_sum<3>(v, sh);
     if (threadIdx.x == 0)
@@ -378,8 +380,8 @@ __device__ inline double padded_value(const typename Store<MODE>::S *st,
     else if (z >= nz) { z -= nz; jump = -pin; }
     const int i = x + nx * y + nxy * z;
     if (mask && mask[i] != 0) return 0.0;
-    if (f == 0) return (double)ST::ld_p(st[i], z, q) + jump;
-    return (double)ST::ld_u(st[f * N + i], q);
+    if (f == 0) return (double)ST::ld_p(st[StateLayout<MODE>::at(0, i, N)], z, q) + jump;
+    return (double)ST::ld_u(st[StateLayout<MODE>::at(f, i, N)], q);
 }
 
 template <int MODE>
@@ -502,10 +504,11 @@ __global__ __launch_bounds__(256) void k_moments(
     for (int z = 0; z < nz; z++) {
         const int i = c + nxy * z;
         if (mask && mask[i] != 0) continue;
-        const double p = ST::ld_p(st[i], z, q);
-        const double u = ST::ld_u(st[N + i], q);
-        const double v = ST::ld_u(st[2 * N + i], q);
-        const double w = ST::ld_u(st[3 * N + i], q);
+        using L = StateLayout<MODE>;
+        const double p = ST::ld_p(st[L::at(0, i, N)], z, q);
+        const double u = ST::ld_u(st[L::at(1, i, N)], q);
+        const double v = ST::ld_u(st[L::at(2, i, N)], q);
+        const double w = ST::ld_u(st[L::at(3, i, N)], q);
         s[0] += p; s[1] += u; s[2] += v; s[3] += w;
         s[4] += p * p; s[5] += u * u; s[6] += v * v; s[7] += w * w;
         s[8] += u * v; s[9] += u * w; s[10] += v * w;

@@ -63,6 +63,7 @@ struct Store;
 
 template <>
 struct Store<FSLBM_F64> {
+    static constexpr int mode = FSLBM_F64;
     using S = double;
     using R = double;
     __host__ __device__ static R ld_p(S v, int, const Quant &) { return v; }
@@ -76,6 +77,7 @@ struct Store<FSLBM_F64> {
 
 template <>
 struct Store<FSLBM_F32> {
+    static constexpr int mode = FSLBM_F32;
     using S = float;
     using R = float;
     __host__ __device__ static R ld_p(S v, int, const Quant &) { return v; }
@@ -87,27 +89,29 @@ struct Store<FSLBM_F32> {
     __device__ static S st_u_sr(R v, const Quant &, uint32_t) { return v; }
 };
 
-// IEEE binary16 storage of p - (pbase + pslope*z) and of u, scaled.
+// IEEE binary16 storage of (p - (pbase + pslope*z)) / sp and of u itself.
+// The velocity is stored unscaled: a binary16 value keeps 11 significant
+// bits whatever its scale, |u| < 65504 never overflows, and below 2^-14
+// (6.1e-5) the subnormals keep an absolute precision of 2^-24 (6e-8), far
+// below the relative error of any velocity that matters to the flow.  So
+// decoding and encoding a velocity is a conversion alone.  The pressure is
+// stored as its deviation from the mean line, a value far smaller than p,
+// and scaled so that small deviations stay in the normal range.
 template <>
 struct Store<FSLBM_F16S> {
+    static constexpr int mode = FSLBM_F16S;
     using S = _Float16;
     using R = float;
     __host__ __device__ static R ld_p(S v, int z, const Quant &q)
     {
         return (q.pbase + q.pslope * (float)z) + q.sp * (float)v;
     }
-    __host__ __device__ static R ld_u(S v, const Quant &q)
-    {
-        return q.su * (float)v;
-    }
+    __host__ __device__ static R ld_u(S v, const Quant &) { return (float)v; }
     __host__ __device__ static S st_p(R v, int z, const Quant &q)
     {
         return (S)((v - (q.pbase + q.pslope * (float)z)) * q.inv_sp);
     }
-    __host__ __device__ static S st_u(R v, const Quant &q)
-    {
-        return (S)(v * q.inv_su);
-    }
+    __host__ __device__ static S st_u(R v, const Quant &) { return (S)v; }
     // Device stores round stochastically (random word r): the expected
     // stored value is the unrounded one, so per-step increments far below
     // half an FP16 ulp are not lost (the host's initial encode rounds to
@@ -122,36 +126,45 @@ struct Store<FSLBM_F16S> {
     }
     __device__ static S st_p_sr(R v, int z, const Quant &q, uint32_t r)
     {
-        return sr((v - (q.pbase + q.pslope * (float)z)) * q.inv_sp, r);
+        return sr(enc_p(v, z, q), r);
     }
-    __device__ static S st_u_sr(R v, const Quant &q, uint32_t r)
+    __device__ static S st_u_sr(R v, const Quant &, uint32_t r) { return sr(v, r); }
+    // the value a pressure stores before rounding
+    __host__ __device__ static float enc_p(R v, int z, const Quant &q)
     {
-        return sr(v * q.inv_su, r);
+        return (v - (q.pbase + q.pslope * (float)z)) * q.inv_sp;
+    }
+    // Two encoded values rounded into the low and high halves of one
+    // 32-bit word (a pair of the 16-bit state layout), each as sr() rounds
+    // it alone.
+    __device__ static uint32_t sr2(float lo, uint32_t rlo, float hi, uint32_t rhi)
+    {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        h2 t = {0, 0};
+        t = __builtin_amdgcn_cvt_sr_f16_f32(t, lo, rlo, false);
+        t = __builtin_amdgcn_cvt_sr_f16_f32(t, hi, rhi, true);
+        return __builtin_bit_cast(uint32_t, t);
     }
 };
 
-// bfloat16 storage of the same shifted, scaled fields.
+// bfloat16 storage of the same fields (bfloat16 has the exponent range of
+// float, so the velocity is stored unscaled too).
 template <>
 struct Store<FSLBM_BF16S> {
+    static constexpr int mode = FSLBM_BF16S;
     using S = uint16_t;
     using R = float;
     __host__ __device__ static R ld_p(S v, int z, const Quant &q)
     {
         return (q.pbase + q.pslope * (float)z) + q.sp * bf16_bits_to_f32(v);
     }
-    __host__ __device__ static R ld_u(S v, const Quant &q)
-    {
-        return q.su * bf16_bits_to_f32(v);
-    }
+    __host__ __device__ static R ld_u(S v, const Quant &) { return bf16_bits_to_f32(v); }
     __host__ __device__ static S st_p(R v, int z, const Quant &q)
     {
         return f32_to_bf16_bits((v - (q.pbase + q.pslope * (float)z)) *
                                 q.inv_sp);
     }
-    __host__ __device__ static S st_u(R v, const Quant &q)
-    {
-        return f32_to_bf16_bits(v * q.inv_su);
-    }
+    __host__ __device__ static S st_u(R v, const Quant &) { return f32_to_bf16_bits(v); }
     static constexpr bool SR = true;
     __device__ static S sr(float x, uint32_t r)
     {
@@ -162,12 +175,44 @@ struct Store<FSLBM_BF16S> {
     }
     __device__ static S st_p_sr(R v, int z, const Quant &q, uint32_t r)
     {
-        return sr((v - (q.pbase + q.pslope * (float)z)) * q.inv_sp, r);
+        return sr(enc_p(v, z, q), r);
     }
-    __device__ static S st_u_sr(R v, const Quant &q, uint32_t r)
+    __device__ static S st_u_sr(R v, const Quant &, uint32_t r) { return sr(v, r); }
+    __host__ __device__ static float enc_p(R v, int z, const Quant &q)
     {
-        return sr(v * q.inv_su, r);
+        return (v - (q.pbase + q.pslope * (float)z)) * q.inv_sp;
+    }
+    __device__ static uint32_t sr2(float lo, uint32_t rlo, float hi, uint32_t rhi)
+    {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        b2 t = {0, 0};
+        t = __builtin_amdgcn_cvt_sr_bf16_f32(t, lo, rlo, false);
+        t = __builtin_amdgcn_cvt_sr_bf16_f32(t, hi, rhi, true);
+        return __builtin_bit_cast(uint32_t, t);
     }
 };
+
+// Layout of one stored state of N cells.  FP64 and FP32 hold four planes
+// of N values, p, ux, uy, uz.  The 16-bit modes hold two planes of N
+// pairs, (p, ux) and (uy, uz), each pair one 32-bit word, so that one
+// 32-bit access moves two fields of a cell (a wave reads 256 bytes per
+// instruction, as with FP32 fields).  at(f, i, N) is the element index of
+// field f (0..3) of cell i.
+template <int MODE>
+struct StateLayout {
+    static constexpr bool PAIRS = sizeof(typename Store<MODE>::S) == 2;
+    __host__ __device__ static long long at(int f, long long i, long long N)
+    {
+        if constexpr (PAIRS) return (long long)(f >> 1) * 2 * N + 2 * i + (f & 1);
+        else return (long long)f * N + i;
+    }
+};
+// The same for a precision mode known at run time.
+inline long long state_at(int precision, int f, long long i, long long N)
+{
+    return precision == FSLBM_F16S || precision == FSLBM_BF16S
+               ? StateLayout<FSLBM_F16S>::at(f, i, N)
+               : StateLayout<FSLBM_F32>::at(f, i, N);
+}
 
 }  // namespace fslbm

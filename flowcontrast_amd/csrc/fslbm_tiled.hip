@@ -47,6 +47,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <utility>
 
 #include "fslbm_internal.h"
@@ -236,16 +237,6 @@ __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-template <>
-__device__ __forceinline__ _Float16 bload<_Float16>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(_Float16, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
-}
-template <>
-__device__ __forceinline__ uint16_t bload<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
-}
 // With NT, FP32 state is stored non-temporal (cache-policy bit NT):
 // nothing in the launch reads it back, so its lines do not displace the
 // input planes that neighbouring tiles still read from L2 (halo rows and
@@ -379,6 +370,7 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
         arm_col = ax + nx * ay;
     }
 
+    static_assert(!StateLayout<MODE>::PAIRS, "the two-barrier kernel steps four-plane states");
     const S *__restrict__ Pi = in;
     const S *__restrict__ Ui = in + N;
     const S *__restrict__ Vi = in + 2 * N;
@@ -646,10 +638,76 @@ __device__ __forceinline__ void pipe_item(
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = P.nxy;
     const int N = nxy * nz;
 
-    const unsigned fbytes = (unsigned)N * (unsigned)sizeof(S);
-    const __amdgpu_buffer_rsrc_t rP = mkbuf(in, fbytes), rU = mkbuf(in + N, fbytes),
-                                 rV = mkbuf(in + 2 * N, fbytes), rW = mkbuf(in + 3 * N, fbytes);
-    auto poff = [&](int zp) { return (unsigned)zp * (unsigned)nxy * (unsigned)sizeof(S); };
+    // State layout (StateLayout): FP32 four planes of fields, one buffer
+    // resource each; the 16-bit modes two planes of pairs (p, ux) and
+    // (uy, uz), where rP = rU and rV = rW and one 32-bit access moves a
+    // pair.  CB: bytes per cell of one resource.
+    constexpr bool PR = StateLayout<MODE>::PAIRS;
+    constexpr unsigned CB = PR ? 4u : (unsigned)sizeof(S);
+    typedef S S2 __attribute__((ext_vector_type(2)));
+    const unsigned fbytes = (unsigned)N * CB;
+    const __amdgpu_buffer_rsrc_t rP = mkbuf(in, fbytes), rV = mkbuf(in + 2 * N, fbytes);
+    const __amdgpu_buffer_rsrc_t rU = PR ? rP : mkbuf(in + N, fbytes),
+                                 rW = PR ? rV : mkbuf(in + 3 * N, fbytes);
+    auto poff = [&](int zp) { return (unsigned)zp * (unsigned)nxy * CB; };
+    // fields 0, 1 (k = 0) or 2, 3 (k = 1) of the cell at column offset cb
+    // and plane offset so
+    auto ldpair = [&](int k, unsigned cb, unsigned so) __attribute__((always_inline)) -> S2 {
+        S2 v;
+        if constexpr (PR) {
+            v = __builtin_bit_cast(S2, __builtin_amdgcn_raw_buffer_load_b32(k ? rV : rP, cb, so, 0));
+        } else {
+            v.x = bload<S>(k ? rV : rP, cb, so);
+            v.y = bload<S>(k ? rW : rU, cb, so);
+        }
+        return v;
+    };
+    // Exact half-to-float of the low (HI false) or high half of a 32-bit
+    // word by integer operations: the half's sign, exponent and mantissa
+    // moved into place read as a float, times 2^112 (the exponent bias
+    // difference; exact for normal and subnormal halves, as float
+    // denormals are preserved).  The conversion instructions
+    // (v_cvt_f32_f16, v_fma_mix_f32) need hazard padding and cost more
+    // than these operations: the FP16-storage step measured 8% faster.
+    auto h2f = [](uint32_t w, auto hi) __attribute__((always_inline)) -> float {
+        constexpr bool HI = decltype(hi)::value;
+        const uint32_t bits = HI ? ((w >> 3) & 0x0fffe000u) | (w & 0x80000000u)
+                                 : ((w << 13) & 0x0fffe000u) | ((w << 16) & 0x80000000u);
+        return __builtin_bit_cast(float, bits) * 0x1p112f;
+    };
+    using LO_ = std::integral_constant<bool, false>;
+    using HI_ = std::integral_constant<bool, true>;
+    // A corrector arm cell (ux, uy, uz) as the loads bring it: three
+    // fields (FP32) or the pairs (p, ux) and (uy, uz) (16-bit modes).
+    using AV = std::conditional_t<PR, S2, S>;
+    constexpr int NA = PR ? 2 : 3;
+    auto arm_load = [&](AV (&v)[NA], unsigned cb, unsigned so) __attribute__((always_inline)) {
+        if constexpr (PR) {
+            v[0] = ldpair(0, cb, so);
+            v[1] = ldpair(1, cb, so);
+        } else {
+            v[0] = bload<S>(rU, cb, so);
+            v[1] = bload<S>(rV, cb, so);
+            v[2] = bload<S>(rW, cb, so);
+        }
+    };
+    auto arm_cell = [&](const AV (&v)[NA]) __attribute__((always_inline)) -> FdCell<R> {
+        FdCell<R> cc;
+        if constexpr (MODE == FSLBM_F16S) {
+            cc.x = h2f(__builtin_bit_cast(uint32_t, v[0]), HI_{});
+            cc.y = h2f(__builtin_bit_cast(uint32_t, v[1]), LO_{});
+            cc.z = h2f(__builtin_bit_cast(uint32_t, v[1]), HI_{});
+        } else if constexpr (PR) {
+            cc.x = ST::ld_u(v[0].y, P.q);
+            cc.y = ST::ld_u(v[1].x, P.q);
+            cc.z = ST::ld_u(v[1].y, P.q);
+        } else {
+            cc.x = ST::ld_u(v[0], P.q);
+            cc.y = ST::ld_u(v[1], P.q);
+            cc.z = ST::ld_u(v[2], P.q);
+        }
+        return cc;
+    };
 
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -669,9 +727,16 @@ __device__ __forceinline__ void pipe_item(
     auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl) -> Msg<R> {
         R jump;
         const int zp = phys(zl, jump);
-        const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
-        const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
-        return make_msg<R>(pp, ux, uy, uz);
+        if constexpr (MODE == FSLBM_F16S) {
+            auto w = [](S v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); };
+            const R pp = fl ? ((P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(w(rp), LO_{})) + jump
+                            : R(0);
+            return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f(w(rw), LO_{}));
+        } else {
+            const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
+            const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
+            return make_msg<R>(pp, ux, uy, uz);
+        }
     };
     // x pass of a row wave: neighbours by DPP, edge lanes from LDS
     auto xpass = [&](int buf, int ry, int ix, const R (&xv)[10], const R (&cen)[6],
@@ -748,34 +813,82 @@ __device__ __forceinline__ void pipe_item(
         // buffer range check drops the write.  (offset + plane offset
         // stays below 2^32: fields are below 2^31 bytes.)
         const bool store = PERIODIC ? true : (inside && fluid);
-        const unsigned cb_ = col * (unsigned)sizeof(S);
+        const unsigned cb_ = col * CB;
         const unsigned sb_ = (PERIODIC && !inside) ? 0x80000000u : cb_;
-        const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oU = mkbuf(out + N, fbytes),
-                                     oV = mkbuf(out + 2 * N, fbytes), oW = mkbuf(out + 3 * N, fbytes);
-        // queue of raw values: plane q in slot (q - zs + 2 + HF) mod QL
-        // two fields per entry, (p, ux) and (uy, uz): in the 16-bit
-        // storage modes a pair shares one register
-        typedef S S2 __attribute__((ext_vector_type(2)));
-        S2 qa[QL], qb[QL];
+        const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oV = mkbuf(out + 2 * N, fbytes);
+        const __amdgpu_buffer_rsrc_t oU = PR ? oP : mkbuf(out + N, fbytes),
+                                     oW = PR ? oV : mkbuf(out + 3 * N, fbytes);
+        // Queue of raw values, two fields per entry, (p, ux) and (uy, uz):
+        // plane q in slot (q - zs + 2 + HF) mod QR.  FP32 reads every value
+        // from it where used (QR = 12).  The 16-bit modes convert each
+        // stored value once: plane a+DA is decoded at iteration a into
+        // the decoded queue dq (slot (q - zs + 2 + HF) mod DQ), which then
+        // serves the message, the corrector tile, the own values and the z
+        // arms (a half-to-float conversion costs several issue slots, and
+        // read where used each value was converted up to five times).  Its
+        // raw queue holds only the planes in flight: at most
+        // HF + LL - DA + 1 <= 4 of them, so a load never overwrites a plane
+        // not yet decoded.  Both lengths divide the 12-fold unroll, so every
+        // slot index is a constant.
+        constexpr int QR = PR ? 4 : QL;
+        constexpr int DA = PR ? (HF - 2 > 1 ? HF - 2 : 1) : 0;
+        constexpr int DQ = PR ? (HF + DA + 3 <= 6 ? 6 : 12) : 1;
+        static_assert(!PR || HF + LL - DA <= 3, "raw queue too short");
+        S2 qa[QR], qb[QR];
+        R dq[DQ][4];
         auto qload = [&](int q, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(q, jump));
-            S2 pa, pb;
-            pa.x = bload<S>(rP, cb_, so);
-            pa.y = bload<S>(rU, cb_, so);
-            pb.x = bload<S>(rV, cb_, so);
-            pb.y = bload<S>(rW, cb_, so);
-            qa[s] = pa;
-            qb[s] = pb;
+            qa[s] = ldpair(0, cb_, so);
+            qb[s] = ldpair(1, cb_, so);
         };
+        // decode plane q from raw slot s into decoded slot d (pressure
+        // without the jump of a wrapped plane: msg_dec adds it)
+        auto qdecode = [&](int q, int s, int d) __attribute__((always_inline)) {
+            R jump;
+            const int zp = phys(q, jump);
+            if constexpr (MODE == FSLBM_F16S) {
+                const uint32_t wa = __builtin_bit_cast(uint32_t, qa[s]),
+                               wb = __builtin_bit_cast(uint32_t, qb[s]);
+                dq[d][0] = (P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(wa, LO_{});
+                dq[d][1] = h2f(wa, HI_{});
+                dq[d][2] = h2f(wb, LO_{});
+                dq[d][3] = h2f(wb, HI_{});
+            } else {
+                dq[d][0] = ST::ld_p(qa[s].x, zp, P.q);
+                dq[d][1] = ST::ld_u(qa[s].y, P.q);
+                dq[d][2] = ST::ld_u(qb[s].x, P.q);
+                dq[d][3] = ST::ld_u(qb[s].y, P.q);
+            }
+        };
+        auto msg_dec = [&](int d, int zl) __attribute__((always_inline)) -> Msg<R> {
+            R jump;
+            (void)phys(zl, jump);
+            const R pp = fluid ? dq[d][0] + jump : R(0);
+            return make_msg<R>(pp, dq[d][1], dq[d][2], dq[d][3]);
+        };
+        Msg<R> mq[3];
+        if constexpr (PR) {
+            // planes zs-HF .. zs+DA-1 decoded before the loop, the rest of
+            // the look-ahead in flight
 #pragma unroll
-        for (int j = 2; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j);
+            for (int j = 2; j < HF + 2 + DA; j++) {
+                qload(zs - 2 - HF + j, 0);
+                qdecode(zs - 2 - HF + j, 0, j % DQ);
+            }
+#pragma unroll
+            for (int j = HF + 2 + DA; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j % QR);
+            mq[0] = msg_dec((HF + 1) % DQ, zs - 1);
+            mq[1] = msg_dec((HF + 2) % DQ, zs);
+        } else {
+#pragma unroll
+            for (int j = 2; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j);
+            mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, fluid);
+            mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fluid);
+        }
         // the last plane any stage of this chunk reads: corrector arm of
         // the last output plane ze-1
         const int qlast = ze - 1 + HF;
-        Msg<R> mq[3];
-        mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, fluid);
-        mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fluid);
         mq[2] = zero_msg<R>();
         R xv[10], cen[6];
 #pragma unroll
@@ -804,8 +917,11 @@ __device__ __forceinline__ void pipe_item(
             // Planes past qlast feed only results past the chunk: their
             // loads re-read plane qlast (a cache hit) instead of fetching
             // planes the next chunk reads again.
-            qload(a + HF + LL < qlast ? a + HF + LL : qlast, (k + 2 * HF + 2 + LL) % QL);
-            {
+            qload(a + HF + LL < qlast ? a + HF + LL : qlast, (k + 2 * HF + 2 + LL) % QR);
+            if constexpr (PR) {
+                qdecode(a + DA, (k + DA + 2 + HF) % QR, (k + DA + 2 + HF) % DQ);
+                mq[(k + 2) % 3] = msg_dec((k + 3 + HF) % DQ, a + 1);
+            } else {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
                 mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fluid);
             }
@@ -813,10 +929,20 @@ __device__ __forceinline__ void pipe_item(
             if (doC) {  // stage C: plane c = a-2
                 const int c = a - 2;
                 const int cb = c & 1;
-                const int sc = (k + HF) % QL;
+                const int sc = (k + HF) % QR;
+                const int dc = (k + HF) % DQ;
                 constexpr R inv216 = R(1) / R(216);
-                const R ux = ST::ld_u(qa[sc].y, P.q), uy = ST::ld_u(qb[sc].x, P.q),
-                        uz = ST::ld_u(qb[sc].y, P.q);
+                // velocity of field f (1..3) of the plane in queue
+                // position `off` from c (a constant after unrolling)
+                auto U = [&](int f, int off) __attribute__((always_inline)) -> R {
+                    if constexpr (PR) {
+                        return dq[(k + HF + off + DQ) % DQ][f];
+                    } else {
+                        const int sl = (k + HF + off + QL) % QL;
+                        return ST::ld_u(f == 1 ? qa[sl].y : f == 2 ? qb[sl].x : qb[sl].y, P.q);
+                    }
+                };
+                const R ux = U(1, 0), uy = U(2, 0), uz = U(3, 0);
                 // unnormalised predictor sums (x216)
                 const R Ps = R(4) * b0c + (QPp + QPm);
                 const R Xs = R(4) * g0c + (QXp + QXm);
@@ -825,21 +951,19 @@ __device__ __forceinline__ void pipe_item(
                 // wall rule terms: own pressure times the wall weights
                 R w0 = 0, wx = 0, wy = 0;
                 if (!PERIODIC) {
-                    const R ps = fluid ? ST::ld_p(qa[sc].x, c, P.q) : R(0);
+                    R pc;
+                    if constexpr (PR) pc = dq[dc][0];
+                    else pc = ST::ld_p(qa[sc].x, c, P.q);
+                    const R ps = fluid ? pc : R(0);
                     w0 = ps * om0;
                     wx = ps * omx;
                     wy = ps * omy;
                 }
                 const int cy_ = HF + iy, cx_ = HF + ix;
-                const int zp1 = (k + HF + D1) % QL, zm1 = (k + HF - D1) % QL;
-                const int zp2 = (k + HF + 2 * D1) % QL, zm2 = (k + HF - 2 * D1 + QL) % QL;
-                auto UX = [&](int sl) { return ST::ld_u(qa[sl].y, P.q); };
-                auto UY = [&](int sl) { return ST::ld_u(qb[sl].x, P.q); };
-                auto UZ = [&](int sl) { return ST::ld_u(qb[sl].y, P.q); };
                 // corrector arm sums at distance d1 (and 2*d1)
-                const R s1x = xp1.x + xm1.x + yp1.x + ym1.x + UX(zp1) + UX(zm1);
-                const R s1y = xp1.y + xm1.y + yp1.y + ym1.y + UY(zp1) + UY(zm1);
-                const R s1z = xp1.z + xm1.z + yp1.z + ym1.z + UZ(zp1) + UZ(zm1);
+                const R s1x = xp1.x + xm1.x + yp1.x + ym1.x + U(1, D1) + U(1, -D1);
+                const R s1y = xp1.y + xm1.y + yp1.y + ym1.y + U(2, D1) + U(2, -D1);
+                const R s1z = xp1.z + xm1.z + yp1.z + ym1.z + U(3, D1) + U(3, -D1);
                 // u' = tb (u*/216 + wall + g (c0 u + c1 S1 + c2 S2)) + (1 - tb) u
                 R nx_ = ux * P.ku + s1x * P.ks1;
                 R ny_ = uy * P.ku + s1y * P.ks1;
@@ -847,9 +971,9 @@ __device__ __forceinline__ void pipe_item(
                 if constexpr (FD2) {  // order-4 arms (blend != 0)
                     const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
                     const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
-                    nx_ += P.ks2 * (xp2.x + xm2.x + yp2.x + ym2.x + UX(zp2) + UX(zm2));
-                    ny_ += P.ks2 * (xp2.y + xm2.y + yp2.y + ym2.y + UY(zp2) + UY(zm2));
-                    nz_ += P.ks2 * (xp2.z + xm2.z + yp2.z + ym2.z + UZ(zp2) + UZ(zm2));
+                    nx_ += P.ks2 * (xp2.x + xm2.x + yp2.x + ym2.x + U(1, 2 * D1) + U(1, -2 * D1));
+                    ny_ += P.ks2 * (xp2.y + xm2.y + yp2.y + ym2.y + U(2, 2 * D1) + U(2, -2 * D1));
+                    nz_ += P.ks2 * (xp2.z + xm2.z + yp2.z + ym2.z + U(3, 2 * D1) + U(3, -2 * D1));
                 }
                 nx_ += P.tb * wx;
                 ny_ += P.tb * wy;
@@ -861,18 +985,41 @@ __device__ __forceinline__ void pipe_item(
                 const S z0 = S(0);
                 const SrWords rw = ST::SR ? sr_words((uint32_t)c * (uint32_t)nxy + col, skey)
                                           : SrWords{0u, 0u};
-                bstore<NTS>(store ? ST::st_p_sr(np_, c, P.q, rw.a) : z0, oP, sb_, so);
-                bstore<NTS>(store ? ST::st_u_sr(nx_, P.q, rw.a << 16) : z0, oU, sb_, so);
-                bstore<NTS>(store ? ST::st_u_sr(ny_, P.q, rw.b) : z0, oV, sb_, so);
-                bstore<NTS>(store ? ST::st_u_sr(nz_, P.q, rw.b << 16) : z0, oW, sb_, so);
+                // Encoded on every lane, then selected: written as
+                // `store ? encode(..) : 0`, the stochastic-rounding
+                // conversions (which the compiler does not speculate)
+                // became four exec-masked branches per plane that broke
+                // the loop's single scheduling region.
+                if constexpr (PR) {  // two packed pairs, one store each
+                    const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.a, nx_, rw.a << 16);
+                    const uint32_t B = ST::sr2(ny_, rw.b, nz_, rw.b << 16);
+                    __builtin_amdgcn_raw_buffer_store_b32(store ? A : 0u, oP, sb_, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(store ? B : 0u, oV, sb_, so, 0);
+                } else {
+                    const S ep = ST::st_p_sr(np_, c, P.q, rw.a);
+                    const S ex = ST::st_u_sr(nx_, P.q, rw.a << 16);
+                    const S ey = ST::st_u_sr(ny_, P.q, rw.b);
+                    const S ez = ST::st_u_sr(nz_, P.q, rw.b << 16);
+                    bstore<NTS>(store ? ep : z0, oP, sb_, so);
+                    bstore<NTS>(store ? ex : z0, oU, sb_, so);
+                    bstore<NTS>(store ? ey : z0, oV, sb_, so);
+                    bstore<NTS>(store ? ez : z0, oW, sb_, so);
+                }
             }
             if (doB) {  // stage B: plane a-1; corrector tile of a-1
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
-                const int sb = (k + 1 + HF) % QL;
                 FdCell<R> cc;
-                cc.x = ST::ld_u(qa[sb].y, P.q);
-                cc.y = ST::ld_u(qb[sb].x, P.q);
-                cc.z = ST::ld_u(qb[sb].y, P.q);
+                if constexpr (PR) {
+                    const int db = (k + 1 + HF) % DQ;
+                    cc.x = dq[db][1];
+                    cc.y = dq[db][2];
+                    cc.z = dq[db][3];
+                } else {
+                    const int sb = (k + 1 + HF) % QL;
+                    cc.x = ST::ld_u(qa[sb].y, P.q);
+                    cc.y = ST::ld_u(qb[sb].x, P.q);
+                    cc.z = ST::ld_u(qb[sb].y, P.q);
+                }
                 fd[(a - 1) & 1][HF + iy][HF + ix] = cc;
             }
             // stage A: plane a
@@ -899,16 +1046,14 @@ __device__ __forceinline__ void pipe_item(
         const bool inside = x < nx && y >= 0 && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
-        const unsigned cb_ = col * (unsigned)sizeof(S);
+        const unsigned cb_ = col * CB;
         // own column prefetch: plane m in slot (m - zs + 1) mod PD
-        S hp[PD], hx[PD], hy[PD], hz[PD];
+        S2 ha[PD], hb[PD];
         auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(m, jump));
-            hp[s] = bload<S>(rP, cb_, so);
-            hx[s] = bload<S>(rU, cb_, so);
-            hy[s] = bload<S>(rV, cb_, so);
-            hz[s] = bload<S>(rW, cb_, so);
+            ha[s] = ldpair(0, cb_, so);
+            hb[s] = ldpair(1, cb_, so);
         };
         // corrector y arms: HF rows of this side, column ix; plane q in
         // slot (q - zs + 1) mod AD
@@ -916,25 +1061,21 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int r = 0; r < HF; r++) {
             const int fy = hr ? TYP + HF + r : r;
-            acol[r] = (colx(x0 + ix) + nx * coly(y0 - HF + fy)) * (unsigned)sizeof(S);
+            acol[r] = (colx(x0 + ix) + nx * coly(y0 - HF + fy)) * CB;
         }
-        S ar[AD][HF][3];
+        AV ar[AD][HF][NA];
         auto aload = [&](int q, int s) __attribute__((always_inline)) {
             const unsigned so = poff(wrapi(q, nz));
 #pragma unroll
-            for (int r = 0; r < HF; r++) {
-                ar[s][r][0] = bload<S>(rU, acol[r], so);
-                ar[s][r][1] = bload<S>(rV, acol[r], so);
-                ar[s][r][2] = bload<S>(rW, acol[r], so);
-            }
+            for (int r = 0; r < HF; r++) arm_load(ar[s][r], acol[r], so);
         };
 #pragma unroll
         for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(hp[0], hx[0], hy[0], hz[0], zs - 1, fluid);
-        mq[1] = msg_of(hp[1], hx[1], hy[1], hz[1], zs, fluid);
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, fluid);
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, fluid);
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -949,7 +1090,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, fluid);
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -960,11 +1101,7 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
                 for (int r = 0; r < HF; r++) {
                     const int fy = hr ? TYP + HF + r : r;
-                    FdCell<R> cc;
-                    cc.x = ST::ld_u(ar[sa][r][0], P.q);
-                    cc.y = ST::ld_u(ar[sa][r][1], P.q);
-                    cc.z = ST::ld_u(ar[sa][r][2], P.q);
-                    fd[(a - 1) & 1][fy][HF + ix] = cc;
+                    fd[(a - 1) & 1][fy][HF + ix] = arm_cell(ar[sa][r]);
                 }
             }
             aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
@@ -992,15 +1129,13 @@ __device__ __forceinline__ void pipe_item(
         const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
         const bool writer = lane < 2 * RYP;
-        const unsigned cb_ = col * (unsigned)sizeof(S);
-        S hp[PD], hx[PD], hy[PD], hz[PD];
+        const unsigned cb_ = col * CB;
+        S2 ha[PD], hb[PD];
         auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(m, jump));
-            hp[s] = bload<S>(rP, cb_, so);
-            hx[s] = bload<S>(rU, cb_, so);
-            hy[s] = bload<S>(rV, cb_, so);
-            hz[s] = bload<S>(rW, cb_, so);
+            ha[s] = ldpair(0, cb_, so);
+            hb[s] = ldpair(1, cb_, so);
         };
         // x arms: NXA cells, XPL per lane
         int afy[XPL], afx[XPL];
@@ -1014,25 +1149,21 @@ __device__ __forceinline__ void pipe_item(
             const int br = cl / (2 * HF), bc = cl - br * (2 * HF);
             afy[j] = HF + br;
             afx[j] = bc < HF ? bc : TX + bc;
-            acol[j] = (colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j])) * (unsigned)sizeof(S);
+            acol[j] = (colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j])) * CB;
         }
-        S ar[AD][XPL][3];
+        AV ar[AD][XPL][NA];
         auto aload = [&](int q, int s) __attribute__((always_inline)) {
             const unsigned so = poff(wrapi(q, nz));
 #pragma unroll
-            for (int j = 0; j < XPL; j++) {
-                ar[s][j][0] = bload<S>(rU, acol[j], so);
-                ar[s][j][1] = bload<S>(rV, acol[j], so);
-                ar[s][j][2] = bload<S>(rW, acol[j], so);
-            }
+            for (int j = 0; j < XPL; j++) arm_load(ar[s][j], acol[j], so);
         };
 #pragma unroll
         for (int j = 0; j < PD; j++) hload(zs - 1 + j, j);
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(hp[0], hx[0], hy[0], hz[0], zs - 1, fluid);
-        mq[1] = msg_of(hp[1], hx[1], hy[1], hz[1], zs, fluid);
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, fluid);
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, fluid);
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -1041,7 +1172,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;
-                mq[(k + 2) % 3] = msg_of(hp[s1], hx[s1], hy[s1], hz[s1], a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, fluid);
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -1051,11 +1182,7 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
                 for (int j = 0; j < XPL; j++) {
                     if (!aw[j]) continue;
-                    FdCell<R> cc;
-                    cc.x = ST::ld_u(ar[sa][j][0], P.q);
-                    cc.y = ST::ld_u(ar[sa][j][1], P.q);
-                    cc.z = ST::ld_u(ar[sa][j][2], P.q);
-                    fd[(a - 1) & 1][afy[j]][afx[j]] = cc;
+                    fd[(a - 1) & 1][afy[j]][afx[j]] = arm_cell(ar[sa][j]);
                 }
             }
             aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
