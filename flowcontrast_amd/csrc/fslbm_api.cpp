@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -112,6 +113,36 @@ int check_params(fslbm_ctx *c, const fslbm_params *p)
     return FSLBM_OK;
 }
 
+// Host staging buffer of a state transfer: pinned memory, or pageable
+// memory (the copies then stage internally) when a pinned buffer of that
+// size cannot be had, as for domains of several GB.
+struct HostStaging {
+    void *p = nullptr;
+    bool pinned = false;
+    std::vector<unsigned char> pageable;
+    explicit HostStaging(size_t bytes)
+    {
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+            return;
+        }
+        (void)hipGetLastError();  // clear the failed allocation's error
+        p = nullptr;
+        try {
+            pageable.resize(bytes);
+            p = pageable.data();
+        } catch (const std::bad_alloc &) {
+            p = nullptr;
+        }
+    }
+    ~HostStaging()
+    {
+        if (pinned) (void)hipHostFree(p);
+    }
+    HostStaging(const HostStaging &) = delete;
+    HostStaging &operator=(const HostStaging &) = delete;
+};
+
 HostMask host_mask(const fslbm_ctx *c)
 {
     HostMask hm;
@@ -132,16 +163,15 @@ int set_state_impl(fslbm_ctx *c, const T *p, const T *ux, const T *uy,
     if (c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S)
         choose_quant(c->g, hm, c->prm.pin, src, c->q);
     const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
-    void *host = nullptr;
-    HIPCHK(c, hipHostMalloc(&host, bytes, hipHostMallocDefault));
-    encode_state(c->g, hm, c->precision, c->q, src, host);
-    hipError_t e = hipMemcpyAsync(c->state[c->cur], host, bytes,
+    HostStaging host(bytes);
+    if (!host.p) return fail(c, FSLBM_ENOMEM, "no host memory for %zu bytes of state", bytes);
+    encode_state(c->g, hm, c->precision, c->q, src, host.p);
+    hipError_t e = hipMemcpyAsync(c->state[c->cur], host.p, bytes,
                                   hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(c->state[c->cur ^ 1], host, bytes,
+        e = hipMemcpyAsync(c->state[c->cur ^ 1], host.p, bytes,
                            hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipHostFree(host);
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "state upload failed: %s", hipGetErrorString(e));
     c->gauge = 0.0;
@@ -155,16 +185,15 @@ int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
     if (!c) return FSLBM_EINVAL;
     if (!p || !ux || !uy || !uz) return fail(c, FSLBM_EINVAL, "NULL state array");
     const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
-    void *host = nullptr;
-    HIPCHK(c, hipHostMalloc(&host, bytes, hipHostMallocDefault));
-    hipError_t e = hipMemcpyAsync(host, c->state[c->cur], bytes,
+    HostStaging host(bytes);
+    if (!host.p) return fail(c, FSLBM_ENOMEM, "no host memory for %zu bytes of state", bytes);
+    hipError_t e = hipMemcpyAsync(host.p, c->state[c->cur], bytes,
                                   hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) {
         T *dst[4] = {p, ux, uy, uz};
-        decode_state(c->g, host_mask(c), c->precision, c->q, host, dst);
+        decode_state(c->g, host_mask(c), c->precision, c->q, host.p, dst);
     }
-    (void)hipHostFree(host);
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "state download failed: %s", hipGetErrorString(e));
     return FSLBM_OK;
@@ -270,8 +299,8 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
         return FSLBM_EINVAL;
     }
     const long long N = (long long)nx * ny * nz;
-    if (N * 4 >= (1ll << 31)) {
-        g_create_err = "domain too large for 32-bit plane indexing (4*N >= 2^31)";
+    if (N >= (1ll << 31)) {
+        g_create_err = "domain too large: cells are indexed with 32 bits (N < 2^31)";
         return FSLBM_EINVAL;
     }
     if (precision < FSLBM_F64 || precision > FSLBM_BF16S) {

@@ -98,8 +98,8 @@ __global__ __launch_bounds__(256) void k_step_plain(
     if (!PERIODIC && mask[i] != 0) return;  // walls stay 0 in both buffers
 
     using L = StateLayout<MODE>;
-    // field f of cell j (32-bit offsets: 4*N < 2^31)
-    auto F = [&](int f, int j) { return in[(int)L::at(f, j, N)]; };
+    // field f of cell j (64-bit offsets: 4*N may exceed 2^31)
+    auto F = [&](int f, int j) { return in[L::at(f, j, N)]; };
     const R ps = ST::ld_p(F(0, i), z, q);
     const R us = ST::ld_u(F(1, i), q), vs = ST::ld_u(F(2, i), q),
             ws = ST::ld_u(F(3, i), q);
@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_reduce(
     using ST = Store<MODE>;
     __shared__ double sh[256 * 8];
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+    // 64-bit loop index: i + the grid stride may pass 2^31 for N < 2^31
+    for (long long i = blockIdx.x * 256 + threadIdx.x; i < N; i += (long long)gridDim.x * 256) {
         if (mask && mask[i] != 0) continue;
         using L = StateLayout<MODE>;
         const double p = ST::ld_p(st[L::at(0, i, N)], i / nxy, q);

@@ -98,8 +98,8 @@ extern "C" int fslbm_step_noalloc(const fslbm_params *prm, const uint8_t *mask2d
         return fail(FSLBM_EINVAL, "fslbm_step_noalloc takes FSLBM_F32 or FSLBM_F64");
     if (nx < 1 || ny < 1 || nz < 1) return fail(FSLBM_EINVAL, "box dimensions must be positive");
     const long long nxy = (long long)nx * ny, N = nxy * nz;
-    if (N * 4 >= (1ll << 31))
-        return fail(FSLBM_EINVAL, "domain too large for 32-bit plane indexing (4*N >= 2^31)");
+    if (N >= (1ll << 31))
+        return fail(FSLBM_EINVAL, "domain too large: cells are indexed with 32 bits (N < 2^31)");
     if (mask2d && nxy > FSLBM_NOALLOC_MAX_COLUMNS)
         return fail(FSLBM_EINVAL,
                     ("walled box of " + std::to_string(nxy) + " columns exceeds the " +
@@ -119,7 +119,8 @@ extern "C" int fslbm_step_noalloc(const fslbm_params *prm, const uint8_t *mask2d
     g.N = N;
     g.periodic = mask2d == nullptr;
     if (!tiled_supported(g, precision, prm->d1, true))
-        return fail(FSLBM_EINVAL, "fslbm_step_noalloc takes d1 = 1 or 2 and nz >= 16");
+        return fail(FSLBM_EINVAL, "fslbm_step_noalloc takes d1 = 1 or 2, nz >= 16 and fields "
+                                  "below 2^32 bytes (2^31 in a periodic box)");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
         return fail(FSLBM_ENODEV, "no usable HIP device");

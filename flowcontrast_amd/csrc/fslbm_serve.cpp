@@ -302,7 +302,7 @@ long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req, const 
     if (req->nx < 1 || req->ny < 1 || req->nz < 1) return sfail(s, FSLBM_EINVAL, "bad box");
     const long long N = (long long)req->nx * req->ny * req->nz;
     const long long nxy = (long long)req->nx * req->ny;
-    if (N * 4 >= (1ll << 31)) return sfail(s, FSLBM_EINVAL, "request too large");
+    if (N >= (1ll << 31)) return sfail(s, FSLBM_EINVAL, "request too large (N >= 2^31 cells)");
     if (mask)
         for (int z = 1; z < req->nz; z++)
             if (std::memcmp(mask, mask + z * nxy, nxy) != 0)

@@ -373,8 +373,8 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
     static_assert(!StateLayout<MODE>::PAIRS, "the two-barrier kernel steps four-plane states");
     const S *__restrict__ Pi = in;
     const S *__restrict__ Ui = in + N;
-    const S *__restrict__ Vi = in + 2 * N;
-    const S *__restrict__ Wi = in + 3 * N;
+    const S *__restrict__ Vi = in + 2ll * N;
+    const S *__restrict__ Wi = in + 3ll * N;
 
     // logical plane -> (physical plane, pressure jump)
     auto phys = [&](int zl, R &jump) -> int {
@@ -545,9 +545,9 @@ __global__ __launch_bounds__(NT) void k_step_tiled(
                 if (store) {
                     const int go = col + nxy * zo;
                     out[go] = ST::st_p(Ps - P.gauge, zo, P.q);
-                    out[N + go] = ST::st_u((Xs + P.g * Lx) * P.tb + ux * P.cc, P.q);
-                    out[2 * N + go] = ST::st_u((Ys + P.g * Ly) * P.tb + uy * P.cc, P.q);
-                    out[3 * N + go] = ST::st_u((Zs + P.g * Lz) * P.tb + uz * P.cc, P.q);
+                    out[(long long)N + go] = ST::st_u((Xs + P.g * Lx) * P.tb + ux * P.cc, P.q);
+                    out[2ll * N + go] = ST::st_u((Ys + P.g * Ly) * P.tb + uy * P.cc, P.q);
+                    out[3ll * N + go] = ST::st_u((Zs + P.g * Lz) * P.tb + uz * P.cc, P.q);
                 }
             }
         }
@@ -646,9 +646,9 @@ __device__ __forceinline__ void pipe_item(
     constexpr unsigned CB = PR ? 4u : (unsigned)sizeof(S);
     typedef S S2 __attribute__((ext_vector_type(2)));
     const unsigned fbytes = (unsigned)N * CB;
-    const __amdgpu_buffer_rsrc_t rP = mkbuf(in, fbytes), rV = mkbuf(in + 2 * N, fbytes);
+    const __amdgpu_buffer_rsrc_t rP = mkbuf(in, fbytes), rV = mkbuf(in + 2ll * N, fbytes);
     const __amdgpu_buffer_rsrc_t rU = PR ? rP : mkbuf(in + N, fbytes),
-                                 rW = PR ? rV : mkbuf(in + 3 * N, fbytes);
+                                 rW = PR ? rV : mkbuf(in + 3ll * N, fbytes);
     auto poff = [&](int zp) { return (unsigned)zp * (unsigned)nxy * CB; };
     // fields 0, 1 (k = 0) or 2, 3 (k = 1) of the cell at column offset cb
     // and plane offset so
@@ -811,13 +811,14 @@ __device__ __forceinline__ void pipe_item(
         // lanes and waves), so its result may differ in the last bit: its
         // store offset is moved past the end of the buffer, where the
         // buffer range check drops the write.  (offset + plane offset
-        // stays below 2^32: fields are below 2^31 bytes.)
+        // stays below 2^32: a periodic box's fields are below 2^31 bytes,
+        // tiled_supported.)
         const bool store = PERIODIC ? true : (inside && fluid);
         const unsigned cb_ = col * CB;
         const unsigned sb_ = (PERIODIC && !inside) ? 0x80000000u : cb_;
-        const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oV = mkbuf(out + 2 * N, fbytes);
+        const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oV = mkbuf(out + 2ll * N, fbytes);
         const __amdgpu_buffer_rsrc_t oU = PR ? oP : mkbuf(out + N, fbytes),
-                                     oW = PR ? oV : mkbuf(out + 3 * N, fbytes);
+                                     oW = PR ? oV : mkbuf(out + 3ll * N, fbytes);
         // Queue of raw values, two fields per entry, (p, ux) and (uy, uz):
         // plane q in slot (q - zs + 2 + HF) mod QR.  FP32 reads every value
         // from it where used (QR = 12).  The 16-bit modes convert each
@@ -1555,9 +1556,17 @@ long long tiles_cells(const Geo &g, int rows, const std::vector<int> &tiles)
 
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant)
 {
-    (void)precision;
     if (d1 != 1 && d1 != 2) return false;
     if (!g.periodic && !z_invariant) return false;
+    // The pipelined kernel addresses each field (FP32) or pair plane
+    // (16-bit modes) through a buffer resource with 32-bit byte offsets:
+    // below 2^32 bytes, and below 2^31 in a periodic box, whose out-of-box
+    // lanes store at an offset past 2^31 that the range check drops.  The
+    // FP64 kernel indexes with 64-bit field offsets.
+    if (precision != FSLBM_F64) {
+        const unsigned long long bytes = 4ull * (unsigned long long)g.N;
+        if (bytes >= (g.periodic ? (1ull << 31) : (1ull << 32))) return false;
+    }
     // The queue runs up to 2*d1 + 3 planes past the chunk end; a single
     // periodic wrap must reach every such plane.
     if (g.nz < 16) return false;

@@ -91,7 +91,7 @@ def test_serving_stream(orc):
     for c in cases:  # step budgets cut 100-fold: 20 to 200 steps
         c.steps = max(2, c.steps // 100)
     built = {}
-    with Server(d1=2, hbm_budget=32 << 30, quantum=16) as srv:
+    with Server(d1=2, hbm_budget=32 << 30, quantum=64) as srv:
         owner = {}
         for k, c in enumerate(cases):
             prm, mask, st = c.build()
