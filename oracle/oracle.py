@@ -10,11 +10,15 @@ list [p, ux, uy, uz].
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-_SO = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+# FSLBM_ORACLE_SO selects another build of the same sources (the
+# sanitizer build of tests/test_oracle_sanitized.py).
+_SO = Path(os.environ.get("FSLBM_ORACLE_SO",
+                          Path(__file__).resolve().parent / "_build" / "liboracle.so"))
 _lib = None
 
 
