@@ -39,6 +39,7 @@ struct fslbm_ctx {
     bool z_invariant = false;  // every z plane of the mask is the same
     uint8_t *mask2d = nullptr; // that plane (device), for the tiled kernel
     void *omega = nullptr;     // per-column wall weights (device)
+    uint16_t *wcode = nullptr; // per-cell wall codes of a z-varying mask (device)
     int *tiles = nullptr;      // tiles holding fluid (device; nullptr: all)
     int ntiles = 0;
     long long tile_cells = 0;  // cells of those tiles
@@ -199,6 +200,15 @@ int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
     return FSLBM_OK;
 }
 
+// The tiled kernel's inputs besides the state, with z planes per work item
+// `chunk` (0: default)
+TiledAux tiled_aux(const fslbm_ctx *c, int chunk)
+{
+    TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, chunk, c->rows, c->stream};
+    aux.wcode = c->wcode;
+    return aux;
+}
+
 bool tiled_ok(const fslbm_ctx *c, int d1)
 {
     return tiled_supported(c->g, c->precision, d1, c->z_invariant);
@@ -230,8 +240,7 @@ int launch_one_step(fslbm_ctx *c, double gauge)
     if (c->kernel == FSLBM_KERNEL_TILED && !tiled_ok(c, c->prm.d1))
         return fail(c, FSLBM_EINVAL, "tiled kernel does not support this case");
     if (use_tiled(c)) {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, c->chunk, c->rows, c->stream};
-        e = launch_step_tiled(c->g, c->precision, in, out, aux, a);
+        e = launch_step_tiled(c->g, c->precision, in, out, tiled_aux(c, c->chunk), a);
     } else
         e = launch_step_plain(c->g, c->precision, in, out, c->mask, a, c->stream);
     if (e != hipSuccess)
@@ -275,6 +284,7 @@ void fslbm_destroy(fslbm_t c)
     if (c->mask) (void)hipFree(c->mask);
     if (c->mask2d) (void)hipFree(c->mask2d);
     if (c->omega) (void)hipFree(c->omega);
+    if (c->wcode) (void)hipFree(c->wcode);
     if (c->tiles) (void)hipFree(c->tiles);
     if (c->partials) (void)hipFree(c->partials);
     if (c->red) (void)hipFree(c->red);
@@ -314,7 +324,6 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
     int rc = check_params(c, prm);
     if (rc) { g_create_err = c->err; delete c; return rc; }
     c->prm = *prm;
-    c->rows = solo_rows(precision, c->g, prm->blend != 0.0);
     if (mask) {
         const long long nxy = (long long)nx * ny;
         for (long long i = 0; i < N; i++) {
@@ -338,7 +347,11 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
             }
         }
         c->hmask.assign(mask, mask + N);
+        c->z_invariant = true;
+        for (int z = 1; z < nz && c->z_invariant; z++)
+            c->z_invariant = std::memcmp(mask, mask + z * nxy, nxy) == 0;
     }
+    c->rows = solo_rows(precision, c->g, prm->blend != 0.0, mask && !c->z_invariant);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         g_create_err = "no HIP device visible";
@@ -370,9 +383,6 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
         ok = alloc((void **)&c->mask, (size_t)N) &&
              hipMemcpy(c->mask, mask, (size_t)N, hipMemcpyHostToDevice) == hipSuccess;
         const size_t nxy = (size_t)nx * ny;
-        c->z_invariant = true;
-        for (int z = 1; z < nz && c->z_invariant; z++)
-            c->z_invariant = std::memcmp(mask, mask + z * nxy, nxy) == 0;
         if (ok && c->z_invariant) {
             ok = alloc((void **)&c->mask2d, nxy) &&
                  hipMemcpy(c->mask2d, mask, nxy, hipMemcpyHostToDevice) == hipSuccess;
@@ -387,8 +397,21 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
                      hipMemcpy(c->omega, om.data(), om.size() * sizeof(float),
                                hipMemcpyHostToDevice) == hipSuccess;
             }
-            const std::vector<int> tl = active_tiles(c->g, c->rows, mask);
-            if (ok && !tl.empty()) {
+        }
+        if (ok && !c->z_invariant && precision != FSLBM_F64) {
+            // wall codes for the tiled kernel's variant for such masks
+            ok = alloc((void **)&c->wcode, 2 * (size_t)N) &&
+                 launch_wall_codes(c->g, c->mask, c->wcode, c->stream) == hipSuccess &&
+                 hipStreamSynchronize(c->stream) == hipSuccess;
+        }
+        if (ok && (c->z_invariant || c->wcode)) {
+            // tiles whose columns hold a fluid cell in some plane
+            std::vector<uint8_t> proj(mask, mask + nxy);
+            for (int z = 1; z < nz && !c->z_invariant; z++)
+                for (size_t i = 0; i < nxy; i++)
+                    if (mask[z * nxy + i] == 0) proj[i] = 0;
+            const std::vector<int> tl = active_tiles(c->g, c->rows, proj.data());
+            if (!tl.empty()) {
                 ok = alloc((void **)&c->tiles, tl.size() * sizeof(int)) &&
                      hipMemcpy(c->tiles, tl.data(), tl.size() * sizeof(int),
                                hipMemcpyHostToDevice) == hipSuccess;
@@ -481,13 +504,12 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
     double bms = 1e300;
     // untimed warm-up, so that the first candidate is not timed on an idle GPU
     {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, cands[0], c->rows, c->stream};
+        const TiledAux aux = tiled_aux(c, cands[0]);
         for (int t = 0; t < 2 * trial_steps; t++)
             HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
     }
     for (int cand : cands) {
-        TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, std::min(cand, c->g.nz), c->rows,
-                     c->stream};
+        const TiledAux aux = tiled_aux(c, std::min(cand, c->g.nz));
         // warm-up launch, then timed launches
         HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));

@@ -83,7 +83,14 @@ struct TiledAux {
     int chunk;  // z planes per work item (0: default)
     int rows;
     hipStream_t stream;
+    // per-cell wall codes of a mask that varies along z (launch_wall_codes;
+    // nullptr: z-invariant mask or periodic box, mask2d and omega instead)
+    const uint16_t *wcode = nullptr;
 };
+// Wall codes (2 bytes per cell) of a walled box's mask, for the tiled
+// kernel's variant for masks that vary along z.
+hipError_t launch_wall_codes(const Geo &g, const uint8_t *mask, uint16_t *codes,
+                             hipStream_t s);
 hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
                              void *out, const TiledAux &aux,
                              const StepArgs &a);
@@ -92,15 +99,16 @@ bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant);
 int device_cus();
 // Rows of the tiled kernels' tiles (64 columns wide): 8 for FP64 and BF16
 // storage, 12 for FP32, and for FP16 storage 12 in a walled box whose
-// corrector has no order-4 arms (blend 0), else 8.  A domain keeps the rows
-// it was planned with; a later blend change runs the other corrector
+// corrector has no order-4 arms (blend 0), else 8; 8 in every 32-bit mode
+// for a walled box whose mask varies along z (zvar).  A domain keeps the
+// rows it was planned with; a later blend change runs the other corrector
 // variant on the same tiles.
-int plan_rows(int precision, bool periodic, bool order4);
+int plan_rows(int precision, bool periodic, bool order4, bool zvar = false);
 // Rows of a single domain's tiles: plan_rows, except that an FP32 periodic
 // box (which steps every tile) takes 8 rows where the step-time model
 // rates their best schedule faster than that of 12 (batches keep 12: one
 // launch per kernel class takes one tile shape).
-int solo_rows(int precision, const Geo &g, bool order4);
+int solo_rows(int precision, const Geo &g, bool order4, bool zvar = false);
 // Tile indices (ty * ntx + tx, tiles of 64 x rows) whose cells include a
 // fluid cell of the z-invariant mask plane; empty for a periodic box
 // (every tile).

@@ -80,6 +80,10 @@ constexpr int F16S_WALLED_ROWS = 12;
 // 144.6 us for 440 items of 52 planes.
 constexpr int F32_BOX_ROWS = 8;
 constexpr double F32_BOX_ROWS_COST = 0.79;
+// Masks that vary along z (wall codes per cell) take 8-row tiles in every
+// storage mode: the code queues and the wall-weight arithmetic need more
+// registers than 12-row tiles (15 waves, 128 registers) leave.
+constexpr int M3_ROWS = 8;
 
 template <typename R>
 struct Msg {
@@ -593,15 +597,41 @@ __device__ __forceinline__ float lane_shift(float src, float edge)
         __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
 }
 
+// Per-cell wall codes of a mask that varies along z (k_wall_codes): the
+// wall rule's weights of one cell, omega = sum over its wall neighbours of
+// W_e (1, e) x 216, split by z plane.  For the 3 x 3 in-plane neighbourhood
+// of a cell (weights x36: corner 1, edge 4, centre 16; a neighbour outside
+// the box counts as a wall) the code holds
+//   bits 0-5    S0 = sum of the weights of the walls (centre included)
+//   bit  6      the cell itself is a wall
+//   bits 8-11   Sx + 6, Sx = sum of weight * cx over the walls
+//   bits 12-15  Sy + 6
+// and the kernel combines the planes c-1, c, c+1 with the z weights
+// (1, 4, 1): omega0 = S0(c-1) + 4 S0(c) + S0(c+1) (the centre of plane c is
+// the fluid cell itself), omegax, omegay likewise, and
+// omegaz = S0(c+1) - S0(c-1).  The fields sum without carries in the bytes
+// of a 32-bit word (wall_word): 152, 72 and 72 at most.
+constexpr uint32_t WC_WALL = 0x40u;
+__device__ __forceinline__ uint32_t wall_word(uint32_t code)
+{
+    return (code & 0x0f3fu) | ((code & 0xf000u) << 4) | ((code & WC_WALL) << 24);
+}
+constexpr uint32_t WW_WALL = WC_WALL << 24;
+
 // One work item of the pipelined step: tile `tile` (ty * ntx + tx) over
 // the z planes [zs, ze).  Called by the single-domain kernel and by the
 // batched kernel; all threads of the workgroup pass the same item.
-template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS>
+// M3: the mask varies along z; every wave reads the wall code of each cell
+// it loads (wcode) instead of the column's mask byte (mask2d), and the
+// interior rows build the wall weights from the codes of three planes
+// instead of reading the column's (omega).
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS, bool M3>
 __device__ __forceinline__ void pipe_item(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
     const typename Store<MODE>::R *__restrict__ omega,
+    const uint16_t *__restrict__ wcode,
     const TiledParams<typename Store<MODE>::R> &P, const int tile, const int zs,
     const int ze, const uint32_t skey)
 {
@@ -609,6 +639,7 @@ __device__ __forceinline__ void pipe_item(
     using R = typename ST::R;
     using S = typename ST::S;
     static_assert(sizeof(R) == 4, "pipelined kernel takes 32-bit arithmetic");
+    static_assert(!(M3 && PERIODIC), "a periodic box has no mask");
     // Corrector reach: d1, and 2*d1 when the order-4 arms carry weight
     // (blend != 0).  The z queue holds the planes c-HF .. a+HF+LL of the
     // stage-C plane c = a-2 in 12 slots, so the loads can run up to
@@ -721,6 +752,14 @@ __device__ __forceinline__ void pipe_item(
     };
     auto colx = [&](int v) { return PERIODIC ? modp(v, nx) : clampi(v, nx); };
     auto coly = [&](int v) { return PERIODIC ? modp(v, ny) : clampi(v, ny); };
+    // wall code (M3) of column `col` in plane zl (wrapped): 2 bytes per
+    // cell, below 2^32 bytes as N < 2^31
+    const __amdgpu_buffer_rsrc_t rC = mkbuf(wcode, M3 ? (unsigned)N * 2u : 0u);
+    auto cload = [&](unsigned col, int zl) __attribute__((always_inline)) -> uint32_t {
+        R jump;
+        const unsigned so = (unsigned)phys(zl, jump) * (unsigned)nxy * 2u;
+        return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rC, col * 2u, so, 0);
+    };
     // Wall cells hold zero velocity in every storage mode, so only their
     // pressure (the decoded reference line and the jump) needs masking for
     // the whole message to vanish.
@@ -797,13 +836,22 @@ __device__ __forceinline__ void pipe_item(
         const int ry = iy + 1;
         const bool inside = x < nx && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
-        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        // M3: `fluid` is the column part of the flag (inside the box); each
+        // plane's own flag adds its wall code's
+        const bool fluid = PERIODIC ? true : M3 ? inside : (inside && mask2d[col] == 0);
         R om0 = 0, omx = 0, omy = 0;
-        if (!PERIODIC) {
+        if (!PERIODIC && !M3) {
             om0 = omega[col];
             omx = omega[nxy + col];
             omy = omega[2 * nxy + col];
         }
+        // Wall codes (M3): raw codes of the planes a+1 and a+2 in
+        // cq[(q - zs) & 1], loaded one iteration ahead of the message of
+        // their plane; wall words of the planes a-3 .. a in ww[(q - zs) & 3]
+        uint32_t cq[2] = {0u, 0u}, ww[4] = {0u, 0u, 0u, 0u};
+        auto flc = [&](uint32_t code) __attribute__((always_inline)) {
+            return M3 ? fluid && !(code & WC_WALL) : fluid;
+        };
         // Lanes that own no fluid cell store 0 to their (wall) column: in
         // a walled box an outside lane's clamped column is a wall.  In a
         // periodic box an outside lane computes a wrapped copy of an inside
@@ -862,12 +910,21 @@ __device__ __forceinline__ void pipe_item(
                 dq[d][3] = ST::ld_u(qb[s].y, P.q);
             }
         };
-        auto msg_dec = [&](int d, int zl) __attribute__((always_inline)) -> Msg<R> {
+        auto msg_dec = [&](int d, int zl, bool fl) __attribute__((always_inline)) -> Msg<R> {
             R jump;
             (void)phys(zl, jump);
-            const R pp = fluid ? dq[d][0] + jump : R(0);
+            const R pp = fl ? dq[d][0] + jump : R(0);
             return make_msg<R>(pp, dq[d][1], dq[d][2], dq[d][3]);
         };
+        bool flm = fluid, fl0 = fluid;  // planes zs-1 and zs
+        if constexpr (M3) {
+            const uint32_t cm = cload(col, zs - 1), c0 = cload(col, zs);
+            cq[1] = cload(col, zs + 1);
+            ww[3] = wall_word(cm);
+            ww[0] = wall_word(c0);
+            flm = flc(cm);
+            fl0 = flc(c0);
+        }
         Msg<R> mq[3];
         if constexpr (PR) {
             // planes zs-HF .. zs+DA-1 decoded before the loop, the rest of
@@ -879,13 +936,13 @@ __device__ __forceinline__ void pipe_item(
             }
 #pragma unroll
             for (int j = HF + 2 + DA; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j % QR);
-            mq[0] = msg_dec((HF + 1) % DQ, zs - 1);
-            mq[1] = msg_dec((HF + 2) % DQ, zs);
+            mq[0] = msg_dec((HF + 1) % DQ, zs - 1, flm);
+            mq[1] = msg_dec((HF + 2) % DQ, zs, fl0);
         } else {
 #pragma unroll
             for (int j = 2; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j);
-            mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, fluid);
-            mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fluid);
+            mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, flm);
+            mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fl0);
         }
         // the last plane any stage of this chunk reads: corrector arm of
         // the last output plane ze-1
@@ -915,16 +972,22 @@ __device__ __forceinline__ void pipe_item(
                 yp1 = fd[cb][cy_ + D1][cx_]; ym1 = fd[cb][cy_ - D1][cx_];
             }
             __builtin_amdgcn_sched_barrier(0);
+            // The wall code of plane a+2 is loaded before this iteration's
+            // state loads, so the wait for it next iteration leaves those
+            // in flight.  Codes are read up to plane ze, the last plane a
+            // result of the chunk depends on.
+            if constexpr (M3) cq[k & 1] = cload(col, a + 2 < ze ? a + 2 : ze);
             // Planes past qlast feed only results past the chunk: their
             // loads re-read plane qlast (a cache hit) instead of fetching
             // planes the next chunk reads again.
             qload(a + HF + LL < qlast ? a + HF + LL : qlast, (k + 2 * HF + 2 + LL) % QR);
+            const bool fl1 = flc(cq[(k + 1) & 1]);  // plane a+1
             if constexpr (PR) {
                 qdecode(a + DA, (k + DA + 2 + HF) % QR, (k + DA + 2 + HF) % DQ);
-                mq[(k + 2) % 3] = msg_dec((k + 3 + HF) % DQ, a + 1);
+                mq[(k + 2) % 3] = msg_dec((k + 3 + HF) % DQ, a + 1, fl1);
             } else {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fl1);
             }
             __builtin_amdgcn_sched_barrier(0);
             if (doC) {  // stage C: plane c = a-2
@@ -950,15 +1013,29 @@ __device__ __forceinline__ void pipe_item(
                 const R Ys = QPp - QPm;
                 const R Zs = R(4) * d0c + (QZp + QZm);
                 // wall rule terms: own pressure times the wall weights
-                R w0 = 0, wx = 0, wy = 0;
+                R w0 = 0, wx = 0, wy = 0, wz = 0;
+                bool st_c = store;
                 if (!PERIODIC) {
                     R pc;
                     if constexpr (PR) pc = dq[dc][0];
                     else pc = ST::ld_p(qa[sc].x, c, P.q);
-                    const R ps = fluid ? pc : R(0);
-                    w0 = ps * om0;
-                    wx = ps * omx;
-                    wy = ps * omy;
+                    if constexpr (M3) {
+                        // wall words of the planes c-1, c, c+1
+                        const uint32_t Wm = ww[(k + 1) & 3], Wc = ww[(k + 2) & 3],
+                                       Wp = ww[(k + 3) & 3];
+                        st_c = fluid && !(Wc & WW_WALL);
+                        const uint32_t T = Wm + Wp + 4u * Wc;
+                        const R pw = (st_c ? pc : R(0)) * inv216;
+                        w0 = pw * (R)(T & 0xffu);
+                        wx = pw * ((R)((T >> 8) & 0xffu) - R(36));
+                        wy = pw * ((R)((T >> 16) & 0xffu) - R(36));
+                        wz = pw * ((R)(Wp & 0xffu) - (R)(Wm & 0xffu));
+                    } else {
+                        const R ps = fluid ? pc : R(0);
+                        w0 = ps * om0;
+                        wx = ps * omx;
+                        wy = ps * omy;
+                    }
                 }
                 const int cy_ = HF + iy, cx_ = HF + ix;
                 // corrector arm sums at distance d1 (and 2*d1)
@@ -978,6 +1055,7 @@ __device__ __forceinline__ void pipe_item(
                 }
                 nx_ += P.tb * wx;
                 ny_ += P.tb * wy;
+                if constexpr (M3) nz_ += P.tb * wz;
                 nx_ += Xs * P.kx;
                 ny_ += Ys * P.kx;
                 nz_ += Zs * P.kx;
@@ -994,19 +1072,22 @@ __device__ __forceinline__ void pipe_item(
                 if constexpr (PR) {  // two packed pairs, one store each
                     const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.a, nx_, rw.a << 16);
                     const uint32_t B = ST::sr2(ny_, rw.b, nz_, rw.b << 16);
-                    __builtin_amdgcn_raw_buffer_store_b32(store ? A : 0u, oP, sb_, so, 0);
-                    __builtin_amdgcn_raw_buffer_store_b32(store ? B : 0u, oV, sb_, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(st_c ? A : 0u, oP, sb_, so, 0);
+                    __builtin_amdgcn_raw_buffer_store_b32(st_c ? B : 0u, oV, sb_, so, 0);
                 } else {
                     const S ep = ST::st_p_sr(np_, c, P.q, rw.a);
                     const S ex = ST::st_u_sr(nx_, P.q, rw.a << 16);
                     const S ey = ST::st_u_sr(ny_, P.q, rw.b);
                     const S ez = ST::st_u_sr(nz_, P.q, rw.b << 16);
-                    bstore<NTS>(store ? ep : z0, oP, sb_, so);
-                    bstore<NTS>(store ? ex : z0, oU, sb_, so);
-                    bstore<NTS>(store ? ey : z0, oV, sb_, so);
-                    bstore<NTS>(store ? ez : z0, oW, sb_, so);
+                    bstore<NTS>(st_c ? ep : z0, oP, sb_, so);
+                    bstore<NTS>(st_c ? ex : z0, oU, sb_, so);
+                    bstore<NTS>(st_c ? ey : z0, oV, sb_, so);
+                    bstore<NTS>(st_c ? ez : z0, oW, sb_, so);
                 }
             }
+            // wall word of plane a+1 (its slot held plane a-3, last read
+            // by stage C above)
+            if constexpr (M3) ww[(k + 1) & 3] = wall_word(cq[(k + 1) & 1]);
             if (doB) {  // stage B: plane a-1; corrector tile of a-1
                 xpass((a - 1) & 1, ry, ix, xv, cen, b0c, g0c, d0c);
                 FdCell<R> cc;
@@ -1046,15 +1127,20 @@ __device__ __forceinline__ void pipe_item(
         const int ry = hr ? TYP + 1 : 0;
         const bool inside = x < nx && y >= 0 && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
-        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        const bool fluid = PERIODIC ? true : M3 ? inside : (inside && mask2d[col] == 0);
         const unsigned cb_ = col * CB;
         // own column prefetch: plane m in slot (m - zs + 1) mod PD
         S2 ha[PD], hb[PD];
+        uint32_t hc[PD];  // wall codes (M3)
         auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(m, jump));
+            if constexpr (M3) hc[s] = cload(col, m);
             ha[s] = ldpair(0, cb_, so);
             hb[s] = ldpair(1, cb_, so);
+        };
+        auto hfl = [&](int s) __attribute__((always_inline)) {
+            return M3 ? fluid && !(hc[s] & WC_WALL) : fluid;
         };
         // corrector y arms: HF rows of this side, column ix; plane q in
         // slot (q - zs + 1) mod AD
@@ -1075,8 +1161,8 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, fluid);
-        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, fluid);
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0));
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1));
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -1091,7 +1177,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1));
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -1128,15 +1214,20 @@ __device__ __forceinline__ void pipe_item(
         const int ry = r;
         const bool inside = x >= 0 && x < nx && y >= 0 && y < ny;
         const unsigned col = colx(x) + nx * coly(y);
-        const bool fluid = PERIODIC ? true : (inside && mask2d[col] == 0);
+        const bool fluid = PERIODIC ? true : M3 ? inside : (inside && mask2d[col] == 0);
         const bool writer = lane < 2 * RYP;
         const unsigned cb_ = col * CB;
         S2 ha[PD], hb[PD];
+        uint32_t hc[PD];  // wall codes (M3)
         auto hload = [&](int m, int s) __attribute__((always_inline)) {
             R jump;
             const unsigned so = poff(phys(m, jump));
+            if constexpr (M3) hc[s] = cload(col, m);
             ha[s] = ldpair(0, cb_, so);
             hb[s] = ldpair(1, cb_, so);
+        };
+        auto hfl = [&](int s) __attribute__((always_inline)) {
+            return M3 ? fluid && !(hc[s] & WC_WALL) : fluid;
         };
         // x arms: NXA cells, XPL per lane
         int afy[XPL], afx[XPL];
@@ -1163,8 +1254,8 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, fluid);
-        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, fluid);
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0));
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1));
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -1173,7 +1264,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;
-                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, fluid);
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1));
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -1216,12 +1307,13 @@ __device__ __forceinline__ void pipe_item(
     }
 }
 
-template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS>
+template <int MODE, bool PERIODIC, int D1, bool FD2, int TYP, bool NTS, bool M3>
 __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     const typename Store<MODE>::S *__restrict__ in,
     typename Store<MODE>::S *__restrict__ out,
     const uint8_t *__restrict__ mask2d,
     const typename Store<MODE>::R *__restrict__ omega,
+    const uint16_t *__restrict__ wcode,
     TiledParams<typename Store<MODE>::R> P)
 {
     // XCD-major work order: the 8 XCDs each take a contiguous run
@@ -1230,8 +1322,8 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_pipe(
     if (w >= P.nwork) return;
     int tile, zs, ze;
     item_of(P, w, tile, zs, ze);
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS>(in, out, mask2d, omega, P, tile, zs, ze,
-                                                P.skey);
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS, M3>(in, out, mask2d, omega, wcode, P, tile, zs,
+                                                    ze, P.skey);
 }
 
 // ---------------------------------------------------------------------
@@ -1279,9 +1371,10 @@ __global__ __launch_bounds__((TYP + 3) * 64) void k_step_batch(
     const int par = D.parity ^ flip;
     const uint32_t seed = seeds[2 * it.x + flip];
     if ((it.w & BATCH_LEAD) && threadIdx.x == 0) seeds[2 * it.x + (flip ^ 1)] = seed + 1u;
-    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS>((const S *)D.buf[par], (S *)D.buf[par ^ 1],
-                                           D.mask2d, D.omega, P, it.y, it.z,
-                                           it.w & ~BATCH_LEAD, sr_key(seed));
+    pipe_item<MODE, PERIODIC, D1, FD2, TYP, NTS, false>((const S *)D.buf[par],
+                                                       (S *)D.buf[par ^ 1], D.mask2d, D.omega,
+                                                       nullptr, P, it.y, it.z,
+                                                       it.w & ~BATCH_LEAD, sr_key(seed));
 }
 
 template <int MODE, int TYP, bool NTS>
@@ -1359,7 +1452,7 @@ void fill_params(TiledParams<typename Store<MODE>::R> &P, const Geo &g, const St
     P.skey = k.skey;
 }
 
-template <int MODE, bool PERIODIC, int D1, int TYP, bool NTS>
+template <int MODE, bool PERIODIC, int D1, int TYP, bool NTS, bool M3>
 hipError_t launch_pipe_nts(const void *in, void *out, const TiledAux &aux,
                            const TiledParams<typename Store<MODE>::R> &P, int grid,
                            hipStream_t s)
@@ -1367,24 +1460,24 @@ hipError_t launch_pipe_nts(const void *in, void *out, const TiledAux &aux,
     using S = typename Store<MODE>::S;
     using R = typename Store<MODE>::R;
     if (P.c2 != R(0))
-        k_step_pipe<MODE, PERIODIC, D1, true, TYP, NTS><<<grid, (TYP + 3) * 64, 0, s>>>(
-            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+        k_step_pipe<MODE, PERIODIC, D1, true, TYP, NTS, M3><<<grid, (TYP + 3) * 64, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, aux.wcode, P);
     else  // blend = 0: the order-4 arms have zero weight
-        k_step_pipe<MODE, PERIODIC, D1, false, TYP, NTS><<<grid, (TYP + 3) * 64, 0, s>>>(
-            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
+        k_step_pipe<MODE, PERIODIC, D1, false, TYP, NTS, M3><<<grid, (TYP + 3) * 64, 0, s>>>(
+            (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, aux.wcode, P);
     return hipGetLastError();
 }
 
 // FP32 steps store non-temporal when the domain's state exceeds the
 // last-level cache (stream_stores); the 16-bit modes never do.
-template <int MODE, bool PERIODIC, int D1, int TYP>
+template <int MODE, bool PERIODIC, int D1, int TYP, bool M3>
 hipError_t launch_pipe(const Geo &g, const void *in, void *out, const TiledAux &aux,
                        const TiledParams<typename Store<MODE>::R> &P, int grid, hipStream_t s)
 {
     if constexpr (MODE == FSLBM_F32)
         if (stream_stores(2 * state_bytes(g, MODE)))
-            return launch_pipe_nts<MODE, PERIODIC, D1, TYP, true>(in, out, aux, P, grid, s);
-    return launch_pipe_nts<MODE, PERIODIC, D1, TYP, false>(in, out, aux, P, grid, s);
+            return launch_pipe_nts<MODE, PERIODIC, D1, TYP, true, M3>(in, out, aux, P, grid, s);
+    return launch_pipe_nts<MODE, PERIODIC, D1, TYP, false, M3>(in, out, aux, P, grid, s);
 }
 
 template <int MODE, bool PERIODIC, int D1>
@@ -1398,19 +1491,28 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
     const int grid = 8 * P.per_xcd;
     // FP64 messages would not fit double-buffered in LDS: FP64 runs the
     // two-barrier kernel, the 32-bit-arithmetic modes the pipelined one.
-    if constexpr (MODE == FSLBM_F64)
+    if constexpr (MODE == FSLBM_F64) {
+        if (aux.wcode) return hipErrorInvalidValue;  // z-invariant masks only
         k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
-    else if constexpr (MODE == FSLBM_F16S && !PERIODIC) {
-        if (aux.rows == F16S_WALLED_ROWS)
-            return launch_pipe<MODE, PERIODIC, D1, F16S_WALLED_ROWS>(g, in, out, aux, P, grid, s);
-    } else if constexpr (MODE == FSLBM_F32 && PERIODIC) {
+    }
+    else if constexpr (!PERIODIC) {
+        // a mask that varies along z: wall codes per cell
+        if (aux.wcode) {
+            if (aux.rows != M3_ROWS) return hipErrorInvalidValue;
+            return launch_pipe<MODE, PERIODIC, D1, M3_ROWS, true>(g, in, out, aux, P, grid, s);
+        }
+        if constexpr (MODE == FSLBM_F16S)
+            if (aux.rows == F16S_WALLED_ROWS)
+                return launch_pipe<MODE, PERIODIC, D1, F16S_WALLED_ROWS, false>(g, in, out, aux, P,
+                                                                            grid, s);
+    } else if constexpr (MODE == FSLBM_F32) {
         if (aux.rows == F32_BOX_ROWS)
-            return launch_pipe<MODE, PERIODIC, D1, F32_BOX_ROWS>(g, in, out, aux, P, grid, s);
+            return launch_pipe<MODE, PERIODIC, D1, F32_BOX_ROWS, false>(g, in, out, aux, P, grid, s);
     }
     if constexpr (MODE != FSLBM_F64) {
         if (aux.rows != pipe_rows(MODE)) return hipErrorInvalidValue;
-        return launch_pipe<MODE, PERIODIC, D1, pipe_rows(MODE)>(g, in, out, aux, P, grid, s);
+        return launch_pipe<MODE, PERIODIC, D1, pipe_rows(MODE), false>(g, in, out, aux, P, grid, s);
     }
     return hipGetLastError();
 }
@@ -1471,16 +1573,17 @@ double chunk_cost(int nz, long long ntile, int nch, int cus)
 
 }  // namespace
 
-int plan_rows(int precision, bool periodic, bool order4)
+int plan_rows(int precision, bool periodic, bool order4, bool zvar)
 {
     if (precision == FSLBM_F64) return TY;
+    if (zvar && !periodic) return M3_ROWS;
     if (precision == FSLBM_F16S && !periodic && !order4) return F16S_WALLED_ROWS;
     return pipe_rows(precision);
 }
 
-int solo_rows(int precision, const Geo &g, bool order4)
+int solo_rows(int precision, const Geo &g, bool order4, bool zvar)
 {
-    const int rows = plan_rows(precision, g.periodic != 0, order4);
+    const int rows = plan_rows(precision, g.periodic != 0, order4, zvar);
     if (precision != FSLBM_F32 || !g.periodic) return rows;
     // every tile of a periodic box is stepped: compare the modelled step
     // times of the best schedules on either tile shape
@@ -1557,7 +1660,8 @@ long long tiles_cells(const Geo &g, int rows, const std::vector<int> &tiles)
 bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant)
 {
     if (d1 != 1 && d1 != 2) return false;
-    if (!g.periodic && !z_invariant) return false;
+    // a mask that varies along z: the pipelined kernel's M3 variant
+    if (!g.periodic && !z_invariant && precision == FSLBM_F64) return false;
     // The pipelined kernel addresses each field (FP32) or pair plane
     // (16-bit modes) through a buffer resource with 32-bit byte offsets:
     // below 2^32 bytes, and below 2^31 in a periodic box, whose out-of-box
@@ -1571,6 +1675,43 @@ bool tiled_supported(const Geo &g, int precision, int d1, bool z_invariant)
     // periodic wrap must reach every such plane.
     if (g.nz < 16) return false;
     return true;
+}
+
+namespace {
+
+// Wall codes of every cell of a walled box (the layout above wall_word).
+__global__ void k_wall_codes(const uint8_t *__restrict__ m, int nx, int ny, long long N,
+                             uint16_t *__restrict__ code)
+{
+    const long long nxy = (long long)nx * ny;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        const long long pl = i / nxy * nxy;  // first cell of the plane
+        const int c = (int)(i - pl);
+        const int x = c % nx, y = c / nx;
+        int s0 = 0, sx = 0, sy = 0;
+        for (int cy = -1; cy <= 1; cy++)
+            for (int cx = -1; cx <= 1; cx++) {
+                const int xx = x + cx, yy = y + cy;
+                const bool wall = xx < 0 || yy < 0 || xx >= nx || yy >= ny ||
+                                  m[pl + xx + (long long)nx * yy] != 0;
+                if (!wall) continue;
+                const int w = (cx ? 1 : 4) * (cy ? 1 : 4);  // corner 1, edge 4, centre 16
+                s0 += w;
+                sx += w * cx;
+                sy += w * cy;
+            }
+        code[i] = (uint16_t)(s0 | (m[i] != 0 ? WC_WALL : 0u) | ((sx + 6) << 8) | ((sy + 6) << 12));
+    }
+}
+
+}  // namespace
+
+hipError_t launch_wall_codes(const Geo &g, const uint8_t *mask, uint16_t *codes, hipStream_t s)
+{
+    const long long blocks = std::min<long long>((g.N + 255) / 256, 65536);
+    k_wall_codes<<<(int)blocks, 256, 0, s>>>(mask, g.nx, g.ny, g.N, codes);
+    return hipGetLastError();
 }
 
 hipError_t launch_step_tiled(const Geo &g, int precision, const void *in,
