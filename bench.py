@@ -543,6 +543,69 @@ def bench_small_pipe():
     return out
 
 
+def bench_rough(steps: int, warmup: int, precisions=("f32", "f16s", "bf16s"),
+                plain_steps: int = 20, chunk: int = -1):
+    """A rough-walled pipe (a mask that varies along z) of the headline
+    pipe's size: 256x256x1024, mean radius 120, wall corrugation of 4 cells
+    and 600 seeded bumps, bulk Re 5300; Poiseuille + 5% noise.  Step rate of
+    the tiled kernel (its variant for such masks) in each 32-bit mode and of
+    the plain kernel (FP32), over at least 200 steps after at least 50
+    warm-up steps (the plain kernel over `plain_steps`, none if 0).  The
+    tiled schedule is autotuned per mode unless `chunk` > 0 fixes it."""
+    from flowcontrast_amd import KERNEL_PLAIN, KERNEL_TILED, RunParams, Solver, inputs
+    nx = ny = 256
+    nz, R, amp = 1024, 120.0, 4.0
+    re, ubar = 5300.0, 0.05
+    nu = ubar * 2 * R / re
+    pin = inputs.blasius_pin(nu, nz, re, R)
+    mask, nf = inputs.rough_pipe_mask(nx, ny, nz, R, amp, 600, 5)
+    st = [a.astype(np.float32) for a in inputs.poiseuille_state(mask, R, ubar, pin, 0.05, 3)]
+    prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=0.0)
+    n = mask.size
+    steps, warmup = max(steps, 200), max(warmup, 50)
+    out = {"workload": f"rough_pipe{nx}x{ny}x{nz}_r{int(R)}_amp{int(amp)}_600bumps",
+           "fluid_cells": nf, "steps": steps, "warmup": warmup,
+           "bytes_per_cell_read_by_kernel": "state + 2-byte wall code"}
+    for prec in precisions:
+        with Solver(mask.shape, prm, mask=mask, precision=prec) as s:
+            assert s.kernel == KERNEL_TILED
+            s.set_state(*st)
+            if chunk > 0:
+                s.set_tile_chunk(chunk)
+            elif chunk < 0:
+                s.autotune(50)
+            s.step(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            ms = s.time_steps(steps) / steps
+            s.sync()
+            wall = time.perf_counter() - t0
+            if s.reduce()[7] != 0:
+                raise RuntimeError(f"non-finite values in the {prec} rough-pipe state")
+            B = algorithmic_bytes_per_cell(prec, True)
+            mlups = n * steps / wall / 1e6
+            out[prec] = {"mlups": round(mlups, 1), "kernel_ms": round(ms, 4),
+                         "chunk": s.tile_chunk, "cells_per_launch": s.step_cells,
+                         "roofline_frac": round(B * s.step_cells / (ms * 1e-3) / 1e9
+                                                / HBM_PEAK_GBS, 4)}
+            if prec != "f32" and "f32" in out:
+                out[prec]["vs_f32"] = round(mlups / out["f32"]["mlups"], 3)
+    if plain_steps <= 0 or "f32" not in out:
+        return out
+    with Solver(mask.shape, prm, mask=mask, precision="f32", kernel=KERNEL_PLAIN) as s:
+        s.set_state(*st)
+        s.step(2)
+        s.sync()
+        t0 = time.perf_counter()
+        ms = s.time_steps(plain_steps) / plain_steps
+        s.sync()
+        wall = time.perf_counter() - t0
+        out["plain_f32"] = {"mlups": round(n * plain_steps / wall / 1e6, 1),
+                            "kernel_ms": round(ms, 4), "steps": plain_steps}
+    out["tiled_vs_plain_f32"] = round(out["f32"]["mlups"] / out["plain_f32"]["mlups"], 2)
+    return out
+
+
 def bench_storage_modes(mask, st, prm, steps: int, warmup: int):
     """The 16-bit storage modes (FP32 arithmetic) on the same pipe and
     initial state as the FP32 line: step rate, roofline fraction over
@@ -643,9 +706,13 @@ def main() -> int:
                     help="steps of the CPU baseline sample (about 10 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "pipe", "batch", "serve", "stats", "tgv"],
+                    choices=["all", "pipe", "batch", "serve", "stats", "tgv", "rough"],
                     help="all: the pipe step line with the batched and serving "
                          "results attached")
+    ap.add_argument("--rough-precisions", default="f32,f16s,bf16s",
+                    help="--workload rough: storage modes to time")
+    ap.add_argument("--rough-plain-steps", type=int, default=20,
+                    help="--workload rough: plain-kernel steps timed (0: none)")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10000)
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -698,6 +765,10 @@ def main() -> int:
     if args.workload == "tgv":
         print(json.dumps(bench_tgv(256, args.tgv_steps, 250, 100, ["f64", "f32", "f16s", "bf16s"])),
               flush=True)
+        return 0
+    if args.workload == "rough":
+        print(json.dumps(bench_rough(args.steps, args.warmup, args.rough_precisions.split(","),
+                                     args.rough_plain_steps, args.chunk)), flush=True)
         return 0
     if args.workload == "stats":
         print(json.dumps(bench_stats(args.stats_spinup, args.stats_window, args.stats_every,
@@ -807,6 +878,7 @@ def main() -> int:
             out["storage_modes"] = {"error": f"{type(e).__name__}: {e}"}
     if args.workload == "all":
         for key, fn in (("small_pipe_f64", bench_small_pipe),
+                        ("rough_pipe", lambda: bench_rough(args.steps, args.warmup)),
                         ("taylor_green", lambda: bench_tgv(256, 5000, 250, 100,
                                                           ["f64", "f32", "f16s", "bf16s"])),
                         ("batched", lambda: bench_batch(args.batch_count, args.batch_steps,

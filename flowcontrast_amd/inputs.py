@@ -2,8 +2,8 @@
 
 These wrap the C ABI builders so the CLI, the tests and the benchmark use
 one implementation: pipe.m's mask (pipe.m:38-58), a centred circular pipe
-for the folded layout, the Poiseuille + seeded-noise state and the
-Taylor-Green vortex.  Arrays are shaped (nz, ny, nx), x fastest.
+for the folded layout, a rough-walled pipe, the Poiseuille + seeded-noise
+state and the Taylor-Green vortex.  Arrays are shaped (nz, ny, nx), x fastest.
 """
 from __future__ import annotations
 
@@ -41,6 +41,41 @@ def circle_mask(nx: int, ny: int, nz: int, radius: float):
     m = np.empty(nx * ny * nz, dtype=np.uint8)
     nf = check(lib().fslbm_circle_mask(nx, ny, nz, float(radius), m))
     return m.reshape(nz, ny, nx), nf
+
+
+def rough_pipe_mask(nx: int, ny: int, nz: int, radius: float, amp: float,
+                    nbumps: int, seed: int):
+    """A rough-walled pipe, a mask that varies along z (the method paper
+    validates rough pipes, MethodDoc_withFSLBMpaper.pdf p.6): a centred pipe
+    whose wall radius R + amp sin(3 theta + 2 pi z/nz) cos(4 pi z/nz) varies
+    with angle and z, plus `nbumps` seeded spherical bumps of radius amp to
+    2 amp centred on the nominal wall (periodic in z), and a 2-cell wall
+    frame on the x/y faces.  Returns (mask, number of fluid cells)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    cx, cy = (nx - 1) / 2.0, (ny - 1) / 2.0
+    r = np.hypot(x - cx, y - cy)
+    th = np.arctan2(y - cy, x - cx)
+    frame = np.zeros((ny, nx), dtype=bool)
+    frame[:, :2] = frame[:, -2:] = True
+    frame[:2, :] = frame[-2:, :] = True
+    mask = np.empty((nz, ny, nx), dtype=np.uint8)
+    for z in range(nz):
+        rw = radius + amp * np.sin(3 * th + 2 * math.pi * z / nz) * math.cos(4 * math.pi * z / nz)
+        mask[z] = np.where((r > rw) | frame, 125, 0)
+    for _ in range(nbumps):
+        t = rng.uniform(0, 2 * math.pi)
+        zc = rng.uniform(0, nz)
+        rb = rng.uniform(amp, 2 * amp)
+        bx, by = cx + radius * math.cos(t), cy + radius * math.sin(t)
+        xs = slice(max(0, int(bx - rb)), min(nx, int(bx + rb) + 2))
+        ys = slice(max(0, int(by - rb)), min(ny, int(by + rb) + 2))
+        d2 = (x[ys, xs] - bx) ** 2 + (y[ys, xs] - by) ** 2
+        for z in range(math.floor(zc - rb), math.ceil(zc + rb) + 1):
+            h2 = rb * rb - (z - zc) ** 2
+            if h2 >= 0:
+                mask[z % nz, ys, xs][d2 <= h2] = 125
+    return mask, int(np.count_nonzero(mask == 0))
 
 
 def poiseuille_state(mask, radius: float, ubar: float, pin: float,
