@@ -40,6 +40,8 @@ struct fslbm_ctx {
     uint8_t *mask2d = nullptr; // that plane (device), for the tiled kernel
     void *omega = nullptr;     // per-column wall weights (device)
     uint16_t *wcode = nullptr; // per-cell wall codes of a z-varying mask (device)
+    float *uref = nullptr;     // streamwise reference profile (16-bit modes, device)
+    std::vector<float> huref;  // its host copy
     int *tiles = nullptr;      // tiles holding fluid (device; nullptr: all)
     int ntiles = 0;
     long long tile_cells = 0;  // cells of those tiles
@@ -161,14 +163,20 @@ int set_state_impl(fslbm_ctx *c, const T *p, const T *ux, const T *uy,
     if (!p || !ux || !uy || !uz) return fail(c, FSLBM_EINVAL, "NULL state array");
     const T *src[4] = {p, ux, uy, uz};
     const HostMask hm = host_mask(c);
-    if (c->precision == FSLBM_F16S || c->precision == FSLBM_BF16S)
+    if (c->uref) {  // 16-bit modes
         choose_quant(c->g, hm, c->prm.pin, src, c->q);
+        c->huref = choose_uref(c->g, hm, src);
+    }
     const size_t bytes = storage_bytes(c->precision) * 4 * (size_t)c->g.N;
     HostStaging host(bytes);
     if (!host.p) return fail(c, FSLBM_ENOMEM, "no host memory for %zu bytes of state", bytes);
-    encode_state(c->g, hm, c->precision, c->q, src, host.p);
-    hipError_t e = hipMemcpyAsync(c->state[c->cur], host.p, bytes,
-                                  hipMemcpyHostToDevice, c->stream);
+    encode_state(c->g, hm, c->precision, c->q, src, host.p, c->uref ? c->huref.data() : nullptr);
+    hipError_t e = hipSuccess;
+    if (c->uref)
+        e = hipMemcpyAsync(c->uref, c->huref.data(), sizeof(float) * c->huref.size(),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->state[c->cur], host.p, bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->state[c->cur ^ 1], host.p, bytes,
                            hipMemcpyHostToDevice, c->stream);
@@ -193,7 +201,8 @@ int get_state_impl(fslbm_ctx *c, T *p, T *ux, T *uy, T *uz)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) {
         T *dst[4] = {p, ux, uy, uz};
-        decode_state(c->g, host_mask(c), c->precision, c->q, host.p, dst);
+        decode_state(c->g, host_mask(c), c->precision, c->q, host.p, dst,
+                     c->uref ? c->huref.data() : nullptr);
     }
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "state download failed: %s", hipGetErrorString(e));
@@ -285,6 +294,7 @@ void fslbm_destroy(fslbm_t c)
     if (c->mask2d) (void)hipFree(c->mask2d);
     if (c->omega) (void)hipFree(c->omega);
     if (c->wcode) (void)hipFree(c->wcode);
+    if (c->uref) (void)hipFree(c->uref);
     if (c->tiles) (void)hipFree(c->tiles);
     if (c->partials) (void)hipFree(c->partials);
     if (c->red) (void)hipFree(c->red);
@@ -433,9 +443,20 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
     }
     if (precision == FSLBM_F16S || precision == FSLBM_BF16S) {
         // Zero state: reference 0 and the smallest pressure scale (as
-        // choose_quant gives for a quiet start) until a state is uploaded.
+        // choose_quant gives for a quiet start) until a state is uploaded;
+        // the streamwise reference profile 0 on every column.
         c->q = Quant{1.0f / 32768, 32768.0f, 1.0f, 1.0f, 0.0f,
                      c->g.periodic ? 0.0f : (float)(-prm->pin / nz)};
+        const size_t nxy = (size_t)nx * ny;
+        c->huref.assign(nxy, 0.0f);
+        if (!alloc((void **)&c->uref, sizeof(float) * nxy) ||
+            hipMemsetAsync(c->uref, 0, sizeof(float) * nxy, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            g_create_err = "device allocation failed (streamwise reference profile)";
+            fslbm_destroy(c);
+            return FSLBM_ENOMEM;
+        }
+        c->q.uref = c->uref;
     }
     *out = c;
     return FSLBM_OK;
@@ -600,20 +621,26 @@ struct SnapHeader {
     char magic[8];
     int32_t precision, pad;
     int64_t n;        // lattice cells
-    Quant q;          // scales of the 16-bit storage modes
+    float q[6];       // 16-bit storage scales: sp, inv_sp, su, inv_su, pbase, pslope
     uint32_t steps;   // stochastic-rounding step counter
     uint32_t pad2;
     double gauge;     // pending gauge of the next step
 };
-// Version 2: velocities of the 16-bit modes stored unscaled (su = 1).
-constexpr char SNAP_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'S', 'N', '2'};
+// Version 3: the 16-bit modes store uz as its departure from a streamwise
+// reference profile, whose nx*ny floats follow the stored state.
+constexpr char SNAP_MAGIC[8] = {'F', 'S', 'L', 'B', 'M', 'S', 'N', '3'};
+
+size_t snap_state_bytes(const fslbm_ctx *c)
+{
+    return storage_bytes(c->precision) * 4 * (size_t)c->g.N;
+}
 
 }  // namespace
 
 long long fslbm_snapshot_size(fslbm_t c)
 {
     if (!c) return FSLBM_EINVAL;
-    return (long long)(sizeof(SnapHeader) + storage_bytes(c->precision) * 4 * (size_t)c->g.N);
+    return (long long)(sizeof(SnapHeader) + snap_state_bytes(c) + sizeof(float) * c->huref.size());
 }
 
 int fslbm_snapshot_save(fslbm_t c, void *buf, long long bytes)
@@ -627,12 +654,16 @@ int fslbm_snapshot_save(fslbm_t c, void *buf, long long bytes)
     std::memcpy(h.magic, SNAP_MAGIC, 8);
     h.precision = c->precision;
     h.n = c->g.N;
-    h.q = c->q;
+    const float qv[6] = {c->q.sp, c->q.inv_sp, c->q.su, c->q.inv_su, c->q.pbase, c->q.pslope};
+    std::memcpy(h.q, qv, sizeof qv);
     h.steps = c->steps;
     h.gauge = c->gauge;
     std::memcpy(buf, &h, sizeof h);
-    HIPCHK(c, hipMemcpyAsync((char *)buf + sizeof h, c->state[c->cur], (size_t)need - sizeof h,
-                             hipMemcpyDeviceToHost, c->stream));
+    const size_t sb = snap_state_bytes(c);
+    HIPCHK(c, hipMemcpyAsync((char *)buf + sizeof h, c->state[c->cur], sb, hipMemcpyDeviceToHost,
+                             c->stream));
+    if (!c->huref.empty())
+        std::memcpy((char *)buf + sizeof h + sb, c->huref.data(), sizeof(float) * c->huref.size());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return FSLBM_OK;
 }
@@ -653,15 +684,25 @@ int fslbm_snapshot_load(fslbm_t c, const void *buf, long long bytes)
                     (long long)h.n, bytes, c->precision, (long long)c->g.N);
     if (!std::isfinite(h.gauge)) return fail(c, FSLBM_EINVAL, "non-finite gauge in snapshot");
     // both buffers, as fslbm_set_state: wall cells hold zero in each
-    const size_t sb = (size_t)need - sizeof h;
+    const size_t sb = snap_state_bytes(c);
     const char *src = (const char *)buf + sizeof h;
+    if (!c->huref.empty())
+        std::memcpy(c->huref.data(), src + sb, sizeof(float) * c->huref.size());
     hipError_t e = hipMemcpyAsync(c->state[c->cur], src, sb, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(c->state[c->cur ^ 1], src, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && c->uref)
+        e = hipMemcpyAsync(c->uref, c->huref.data(), sizeof(float) * c->huref.size(),
+                           hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess)
         return fail(c, FSLBM_EHIP, "snapshot upload failed: %s", hipGetErrorString(e));
-    c->q = h.q;
+    c->q.sp = h.q[0];
+    c->q.inv_sp = h.q[1];
+    c->q.su = h.q[2];
+    c->q.inv_su = h.q[3];
+    c->q.pbase = h.q[4];
+    c->q.pslope = h.q[5];
     c->steps = h.steps;
     c->gauge = h.gauge;
     return FSLBM_OK;
@@ -779,7 +820,10 @@ int fslbm_probe(fslbm_t c, int x, int y, int z, double out[4])
         using ST = decltype(tag);
         typename ST::S v;
         std::memcpy(&v, raw[f], sizeof v);
-        return f == 0 ? (double)ST::ld_p(v, z, c->q) : (double)ST::ld_u(v, c->q);
+        if (f == 0) return (double)ST::ld_p(v, z, c->q);
+        if (f == 3 && c->uref)  // as the kernels decode: one float sum
+            return (double)(ST::ld_u(v, c->q) + c->huref[(size_t)(i % ((long long)c->g.nx * c->g.ny))]);
+        return (double)ST::ld_u(v, c->q);
     };
     for (int f = 0; f < 4; f++) {
         double v = 0;

@@ -35,6 +35,8 @@ struct Domain {
     int ntiles = 0;
     std::vector<int> htiles;  // host copy (empty: all tiles)
     std::vector<uint8_t> hmask2d;
+    float *uref = nullptr;     // streamwise reference profile (16-bit modes)
+    std::vector<float> huref;  // its host copy
     size_t bytes = 0;
     int parity = 0;  // buffer holding the current state
     long steps = 0;
@@ -425,7 +427,8 @@ size_t domain_bytes(int precision, int nx, int ny, int nz, bool masked)
     const size_t N = (size_t)nx * ny * nz, nxy = (size_t)nx * ny;
     const size_t rb = precision == FSLBM_F64 ? 8 : 4;
     const size_t tiles = (size_t)((nx + 63) / 64) * ((ny + 7) / 8);  // bound on the tile list
-    return 2 * 4 * N * storage_bytes(precision) +
+    const size_t uref = precision == FSLBM_F16S || precision == FSLBM_BF16S ? 4 * nxy : 0;
+    return 2 * 4 * N * storage_bytes(precision) + uref +
            (masked ? nxy + 3 * nxy * rb + tiles * sizeof(int) : 0);
 }
 
@@ -477,9 +480,13 @@ int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, 
     hm.m = mask ? pd.mask2d.data() : nullptr;
     hm.nxy = nxy;
     hm.plane = true;
-    if (b->precision != FSLBM_F32) choose_quant(g, hm, prm->pin, src, pd.q);
+    if (b->precision != FSLBM_F32) {
+        choose_quant(g, hm, prm->pin, src, pd.q);
+        pd.uref = choose_uref(g, hm, src);
+    }
     pd.host.resize(4 * (size_t)N * storage_bytes(b->precision));
-    encode_state(g, hm, b->precision, pd.q, src, pd.host.data());
+    encode_state(g, hm, b->precision, pd.q, src, pd.host.data(),
+                 pd.uref.empty() ? nullptr : pd.uref.data());
     if (mask) {
         pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
         // planned with the rows of the domain's kernel class (batch_fill_dom
@@ -512,15 +519,23 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
     d.bytes = bytes;
     d.hmask2d = std::move(pd.mask2d);
     d.htiles = pd.tiles;
+    d.huref = pd.uref;
     const size_t sbytes = pd.host.size();
     auto cleanup = [&]() {
         for (void *q : d.buf) if (q) (void)hipFree(q);
         if (d.mask2d) (void)hipFree(d.mask2d);
         if (d.omega) (void)hipFree(d.omega);
         if (d.tiles) (void)hipFree(d.tiles);
+        if (d.uref) (void)hipFree(d.uref);
     };
     bool ok = hipMalloc(&d.buf[0], sbytes) == hipSuccess &&
               hipMalloc(&d.buf[1], sbytes) == hipSuccess;
+    if (ok && !d.huref.empty()) {
+        ok = hipMalloc((void **)&d.uref, sizeof(float) * d.huref.size()) == hipSuccess &&
+             hipMemcpyAsync(d.uref, pd.uref.data(), sizeof(float) * pd.uref.size(),
+                            hipMemcpyHostToDevice, s) == hipSuccess;
+        d.q.uref = d.uref;
+    }
     if (ok && !g.periodic) {
         ok = hipMalloc((void **)&d.mask2d, nxy) == hipSuccess &&
              hipMalloc(&d.omega, pd.omega.size() * sizeof(float)) == hipSuccess &&
@@ -545,6 +560,7 @@ int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync)
         pd.host = std::vector<unsigned char>();
         pd.omega = std::vector<float>();
         pd.tiles = std::vector<int>();
+        pd.uref = std::vector<float>();
     }  // else the caller keeps them until it has synchronised s
     // reuse a free slot
     int slot = -1;
@@ -619,6 +635,7 @@ void fslbm_batch_destroy(fslbm_batch_t b)
         if (d.mask2d) (void)hipFree(d.mask2d);
         if (d.omega) (void)hipFree(d.omega);
         if (d.tiles) (void)hipFree(d.tiles);
+        if (d.uref) (void)hipFree(d.uref);
     }
     if (b->d_doms) (void)hipFree(b->d_doms);
     if (b->d_seeds) (void)hipFree(b->d_seeds);
@@ -667,9 +684,11 @@ int fslbm_batch_remove(fslbm_batch_t b, int slot)
     if (d.mask2d) (void)hipFree(d.mask2d);
     if (d.omega) (void)hipFree(d.omega);
     if (d.tiles) (void)hipFree(d.tiles);
+    if (d.uref) (void)hipFree(d.uref);
     d.mask2d = nullptr;
     d.omega = nullptr;
     d.tiles = nullptr;
+    d.uref = nullptr;
     b->used -= d.bytes;
     d = Domain();
     b->dirty = true;
@@ -702,7 +721,8 @@ int fslbm_batch_get_state(fslbm_batch_t b, int slot, double *p, double *ux, doub
     hm.nxy = (long long)d.g.nx * d.g.ny;
     hm.plane = true;
     double *dst[4] = {p, ux, uy, uz};
-    decode_state(d.g, hm, b->precision, d.q, host.data(), dst);
+    decode_state(d.g, hm, b->precision, d.q, host.data(), dst,
+                 d.huref.empty() ? nullptr : d.huref.data());
     return FSLBM_OK;
 }
 
@@ -851,6 +871,7 @@ int batch_copy_out(fslbm_batch *b, int slot, HostResult &out, hipStream_t s)
     out.precision = b->precision;
     out.q = d.q;
     out.mask2d = d.hmask2d;
+    out.uref = d.huref;
     out.steps = d.steps;
     out.raw.resize(sbytes);
     BCHK(b, hipMemcpyAsync(out.raw.data(), d.buf[d.held_cur], sbytes, hipMemcpyDeviceToHost, s));
@@ -871,6 +892,7 @@ int batch_take(fslbm_batch *b, int slot, HostResult &out)
     out.precision = b->precision;
     out.q = d.q;
     out.mask2d = d.hmask2d;
+    out.uref = d.huref;
     out.steps = d.steps;
     out.raw.resize(sbytes);
     BCHK(b, hipMemcpyAsync(out.raw.data(), d.buf[cur], sbytes, hipMemcpyDeviceToHost, b->stream));

@@ -19,11 +19,18 @@ struct Geo {
 };
 
 // Dequantisation of the 16-bit shifted storage modes:
-//   p = pbase + pslope*z + sp*h_p,   u = h_u.
+//   p  = pbase + pslope*z + sp*h_p,
+//   ux = h_ux,  uy = h_uy,  uz = h_uz + uref[x + nx*y].
 // sp is a power of two (exact scaling); su is 1 (velocities are stored
-// unscaled) and kept so that snapshots carry the whole record.
+// unscaled) and kept so that snapshots carry the whole record.  uref is
+// the streamwise reference profile, one float per column of the box in
+// device memory (choose_uref): the stored uz is the flow's departure from
+// it, and a rounding error relative to that departure is correspondingly
+// smaller.  uref is nullptr in FP64 and FP32 (and reads as 0); it is 0 on
+// every column that holds a wall cell, so walls still decode to 0.
 struct Quant {
     float sp, inv_sp, su, inv_su, pbase, pslope;
+    const float *uref = nullptr;  // device address (never dereferenced on the host)
 };
 
 // Per-launch scalars of one step (FSLBM.cu:231,269,599,623-624), in the

@@ -100,9 +100,16 @@ __global__ __launch_bounds__(256) void k_step_plain(
     using L = StateLayout<MODE>;
     // field f of cell j (64-bit offsets: 4*N may exceed 2^31)
     auto F = [&](int f, int j) { return in[L::at(f, j, N)]; };
+    // uz of cell j in column c: the 16-bit modes add the column's
+    // streamwise reference
+    auto UZ = [&](int j, int c) -> R {
+        const R w = ST::ld_u(F(3, j), q);
+        if constexpr (ST::SR) return q.uref ? w + q.uref[c] : w;
+        else return w;
+    };
+    const int col = x + nx * y;
     const R ps = ST::ld_p(F(0, i), z, q);
-    const R us = ST::ld_u(F(1, i), q), vs = ST::ld_u(F(2, i), q),
-            ws = ST::ld_u(F(3, i), q);
+    const R us = ST::ld_u(F(1, i), q), vs = ST::ld_u(F(2, i), q), ws = UZ(i, col);
 
     R rt = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll
@@ -125,7 +132,7 @@ __global__ __launch_bounds__(256) void k_step_plain(
                 const bool wall = !PERIODIC && mask[j] != 0;
                 const R pn = wall ? ps : ST::ld_p(F(0, j), zz, q) + jump;
                 const R un = ST::ld_u(F(1, j), q), vn = ST::ld_u(F(2, j), q),
-                        wn = ST::ld_u(F(3, j), q);
+                        wn = UZ(j, xx + nx * yy);
                 const R e_u = (R)cx * un + (R)cy * vn + (R)cz * wn;
                 const R t = (R)3 * e_u;
                 const R uu = (R)3 * (un * un + vn * vn + wn * wn);
@@ -158,15 +165,15 @@ __global__ __launch_bounds__(256) void k_step_plain(
         }
         zm = zm < 0 ? zm + nz : zm;
         zp = zp >= nz ? zp - nz : zp;
-        const int o[6] = {xp + nx * y + nxy * z, xm + nx * y + nxy * z,
-                          x + nx * yp + nxy * z, x + nx * ym + nxy * z,
-                          x + nx * y + nxy * zp, x + nx * y + nxy * zm};
+        const int oc[6] = {xp + nx * y, xm + nx * y, x + nx * yp, x + nx * ym, col, col};
+        const int o[6] = {oc[0] + nxy * z, oc[1] + nxy * z, oc[2] + nxy * z, oc[3] + nxy * z,
+                          col + nxy * zp, col + nxy * zm};
         R sx = 0, sy = 0, sz = 0;
 #pragma unroll
         for (int t = 0; t < 6; t++) {
             sx += ST::ld_u(F(1, o[t]), q);
             sy += ST::ld_u(F(2, o[t]), q);
-            sz += ST::ld_u(F(3, o[t]), q);
+            sz += UZ(o[t], oc[t]);
         }
         Lx += cr * sx;
         Ly += cr * sy;
@@ -178,7 +185,10 @@ __global__ __launch_bounds__(256) void k_step_plain(
     out[L::at(0, i, N)] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
     out[L::at(1, i, N)] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
     out[L::at(2, i, N)] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);
-    out[L::at(3, i, N)] = ST::st_u_sr((az + g * Lz) * k.c + ws * k.cc, q, rw.b << 16);
+    R wn = (az + g * Lz) * k.c + ws * k.cc;
+    if constexpr (ST::SR)
+        if (q.uref) wn -= q.uref[col];
+    out[L::at(3, i, N)] = ST::st_u_sr(wn, q, rw.b << 16);
 }
 
 template <int MODE>
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(256) void k_reduce(
         const double p = ST::ld_p(st[L::at(0, i, N)], i / nxy, q);
         const double u = ST::ld_u(st[L::at(1, i, N)], q);
         const double w = ST::ld_u(st[L::at(2, i, N)], q);
-        const double z = ST::ld_u(st[L::at(3, i, N)], q);
+        const double z = ld_uz<MODE>(st[L::at(3, i, N)], q, i % nxy);
         v[0] += 1.0;
         v[1] += p;
         v[2] += u;
@@ -328,7 +338,7 @@ __global__ __launch_bounds__(256) void k_monitor(
         if (mask && mask[i] != 0) continue;
         v[0] += 1.0;
         v[1] += ST::ld_p(st[StateLayout<MODE>::at(0, i, N)], 0, q);
-        v[2] += ST::ld_u(st[StateLayout<MODE>::at(3, i, N)], q);
+        v[2] += ld_uz<MODE>(st[StateLayout<MODE>::at(3, i, N)], q, i);
     }
     block_sum<3>(v, sh);
     if (threadIdx.x == 0)
@@ -382,6 +392,7 @@ __device__ inline double padded_value(const typename Store<MODE>::S *st,
     const int i = x + nx * y + nxy * z;
     if (mask && mask[i] != 0) return 0.0;
     if (f == 0) return (double)ST::ld_p(st[StateLayout<MODE>::at(0, i, N)], z, q) + jump;
+    if (f == 3) return (double)ld_uz<MODE>(st[StateLayout<MODE>::at(3, i, N)], q, x + nx * y);
     return (double)ST::ld_u(st[StateLayout<MODE>::at(f, i, N)], q);
 }
 
@@ -509,7 +520,7 @@ __global__ __launch_bounds__(256) void k_moments(
         const double p = ST::ld_p(st[L::at(0, i, N)], z, q);
         const double u = ST::ld_u(st[L::at(1, i, N)], q);
         const double v = ST::ld_u(st[L::at(2, i, N)], q);
-        const double w = ST::ld_u(st[L::at(3, i, N)], q);
+        const double w = ld_uz<MODE>(st[L::at(3, i, N)], q, c);
         s[0] += p; s[1] += u; s[2] += v; s[3] += w;
         s[4] += p * p; s[5] += u * u; s[6] += v * v; s[7] += w * w;
         s[8] += u * v; s[9] += u * w; s[10] += v * w;

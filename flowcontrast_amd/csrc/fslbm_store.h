@@ -192,6 +192,17 @@ struct Store<FSLBM_BF16S> {
     }
 };
 
+// uz of a stored value of column `col` (device code): the 16-bit modes
+// add the column's streamwise reference (Quant::uref), once, in float.
+template <int MODE>
+__device__ inline typename Store<MODE>::R ld_uz(typename Store<MODE>::S v, const Quant &q,
+                                                long long col)
+{
+    const typename Store<MODE>::R w = Store<MODE>::ld_u(v, q);
+    if constexpr (Store<MODE>::SR) return q.uref ? w + q.uref[col] : w;
+    else return w;
+}
+
 // Layout of one stored state of N cells.  FP64 and FP32 hold four planes
 // of N values, p, ux, uy, uz.  The 16-bit modes hold two planes of N
 // pairs, (p, ux) and (uy, uz), each pair one 32-bit word, so that one

@@ -722,16 +722,22 @@ __device__ __forceinline__ void pipe_item(
             v[2] = bload<S>(rW, cb, so);
         }
     };
-    auto arm_cell = [&](const AV (&v)[NA]) __attribute__((always_inline)) -> FdCell<R> {
+    // streamwise reference of column c (16-bit modes; 0 in FP32)
+    auto uref_of = [&](unsigned c) __attribute__((always_inline)) -> R {
+        if constexpr (PR) return P.q.uref ? P.q.uref[c] : R(0);
+        else return R(0);
+    };
+    // ur: the cell's column's streamwise reference (16-bit modes)
+    auto arm_cell = [&](const AV (&v)[NA], R ur) __attribute__((always_inline)) -> FdCell<R> {
         FdCell<R> cc;
         if constexpr (MODE == FSLBM_F16S) {
             cc.x = h2f(__builtin_bit_cast(uint32_t, v[0]), HI_{});
             cc.y = h2f(__builtin_bit_cast(uint32_t, v[1]), LO_{});
-            cc.z = h2f(__builtin_bit_cast(uint32_t, v[1]), HI_{});
+            cc.z = h2f(__builtin_bit_cast(uint32_t, v[1]), HI_{}) + ur;
         } else if constexpr (PR) {
             cc.x = ST::ld_u(v[0].y, P.q);
             cc.y = ST::ld_u(v[1].x, P.q);
-            cc.z = ST::ld_u(v[1].y, P.q);
+            cc.z = ST::ld_u(v[1].y, P.q) + ur;
         } else {
             cc.x = ST::ld_u(v[0], P.q);
             cc.y = ST::ld_u(v[1], P.q);
@@ -763,17 +769,20 @@ __device__ __forceinline__ void pipe_item(
     // Wall cells hold zero velocity in every storage mode, so only their
     // pressure (the decoded reference line and the jump) needs masking for
     // the whole message to vanish.
-    auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl) -> Msg<R> {
+    // (ur: the column's streamwise reference, 16-bit modes)
+    auto msg_of = [&](S rp, S ru, S rv, S rw, int zl, bool fl, R ur) -> Msg<R> {
         R jump;
         const int zp = phys(zl, jump);
         if constexpr (MODE == FSLBM_F16S) {
             auto w = [](S v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); };
             const R pp = fl ? ((P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(w(rp), LO_{})) + jump
                             : R(0);
-            return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f(w(rw), LO_{}));
+            return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f(w(rw), LO_{}) + ur);
         } else {
             const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
-            const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q), uz = ST::ld_u(rw, P.q);
+            R uz = ST::ld_u(rw, P.q);
+            if constexpr (PR) uz += ur;
+            const R ux = ST::ld_u(ru, P.q), uy = ST::ld_u(rv, P.q);
             return make_msg<R>(pp, ux, uy, uz);
         }
     };
@@ -852,6 +861,7 @@ __device__ __forceinline__ void pipe_item(
         auto flc = [&](uint32_t code) __attribute__((always_inline)) {
             return M3 ? fluid && !(code & WC_WALL) : fluid;
         };
+        const R urc = uref_of(col);  // the column's streamwise reference
         // Lanes that own no fluid cell store 0 to their (wall) column: in
         // a walled box an outside lane's clamped column is a wall.  In a
         // periodic box an outside lane computes a wrapped copy of an inside
@@ -902,12 +912,12 @@ __device__ __forceinline__ void pipe_item(
                 dq[d][0] = (P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(wa, LO_{});
                 dq[d][1] = h2f(wa, HI_{});
                 dq[d][2] = h2f(wb, LO_{});
-                dq[d][3] = h2f(wb, HI_{});
+                dq[d][3] = h2f(wb, HI_{}) + urc;
             } else {
                 dq[d][0] = ST::ld_p(qa[s].x, zp, P.q);
                 dq[d][1] = ST::ld_u(qa[s].y, P.q);
                 dq[d][2] = ST::ld_u(qb[s].x, P.q);
-                dq[d][3] = ST::ld_u(qb[s].y, P.q);
+                dq[d][3] = ST::ld_u(qb[s].y, P.q) + urc;
             }
         };
         auto msg_dec = [&](int d, int zl, bool fl) __attribute__((always_inline)) -> Msg<R> {
@@ -941,8 +951,8 @@ __device__ __forceinline__ void pipe_item(
         } else {
 #pragma unroll
             for (int j = 2; j < 2 * HF + 2 + LL; j++) qload(zs - 2 - HF + j, j);
-            mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, flm);
-            mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fl0);
+            mq[0] = msg_of(qa[HF + 1].x, qa[HF + 1].y, qb[HF + 1].x, qb[HF + 1].y, zs - 1, flm, urc);
+            mq[1] = msg_of(qa[HF + 2].x, qa[HF + 2].y, qb[HF + 2].x, qb[HF + 2].y, zs, fl0, urc);
         }
         // the last plane any stage of this chunk reads: corrector arm of
         // the last output plane ze-1
@@ -987,7 +997,7 @@ __device__ __forceinline__ void pipe_item(
                 mq[(k + 2) % 3] = msg_dec((k + 3 + HF) % DQ, a + 1, fl1);
             } else {
                 const int s1 = (k + 3 + HF) % QL;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fl1);
+                mq[(k + 2) % 3] = msg_of(qa[s1].x, qa[s1].y, qb[s1].x, qb[s1].y, a + 1, fl1, urc);
             }
             __builtin_amdgcn_sched_barrier(0);
             if (doC) {  // stage C: plane c = a-2
@@ -1071,7 +1081,7 @@ __device__ __forceinline__ void pipe_item(
                 // the loop's single scheduling region.
                 if constexpr (PR) {  // two packed pairs, one store each
                     const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.a, nx_, rw.a << 16);
-                    const uint32_t B = ST::sr2(ny_, rw.b, nz_, rw.b << 16);
+                    const uint32_t B = ST::sr2(ny_, rw.b, nz_ - urc, rw.b << 16);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? A : 0u, oP, sb_, so, 0);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? B : 0u, oV, sb_, so, 0);
                 } else {
@@ -1142,13 +1152,17 @@ __device__ __forceinline__ void pipe_item(
         auto hfl = [&](int s) __attribute__((always_inline)) {
             return M3 ? fluid && !(hc[s] & WC_WALL) : fluid;
         };
+        const R hur = uref_of(col);  // the column's streamwise reference
         // corrector y arms: HF rows of this side, column ix; plane q in
         // slot (q - zs + 1) mod AD
         unsigned acol[HF];
+        R aref[HF];  // their columns' streamwise references
 #pragma unroll
         for (int r = 0; r < HF; r++) {
             const int fy = hr ? TYP + HF + r : r;
-            acol[r] = (colx(x0 + ix) + nx * coly(y0 - HF + fy)) * CB;
+            const unsigned ac = colx(x0 + ix) + nx * coly(y0 - HF + fy);
+            acol[r] = ac * CB;
+            aref[r] = uref_of(ac);
         }
         AV ar[AD][HF][NA];
         auto aload = [&](int q, int s) __attribute__((always_inline)) {
@@ -1161,8 +1175,8 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0));
-        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1));
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0), hur);
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1), hur);
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -1177,7 +1191,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;  // plane a+1
-                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1));
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1), hur);
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -1188,7 +1202,7 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
                 for (int r = 0; r < HF; r++) {
                     const int fy = hr ? TYP + HF + r : r;
-                    fd[(a - 1) & 1][fy][HF + ix] = arm_cell(ar[sa][r]);
+                    fd[(a - 1) & 1][fy][HF + ix] = arm_cell(ar[sa][r], aref[r]);
                 }
             }
             aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
@@ -1229,10 +1243,12 @@ __device__ __forceinline__ void pipe_item(
         auto hfl = [&](int s) __attribute__((always_inline)) {
             return M3 ? fluid && !(hc[s] & WC_WALL) : fluid;
         };
+        const R hur = uref_of(col);  // the column's streamwise reference
         // x arms: NXA cells, XPL per lane
         int afy[XPL], afx[XPL];
         bool aw[XPL];
         unsigned acol[XPL];
+        R aref[XPL];  // their columns' streamwise references
 #pragma unroll
         for (int j = 0; j < XPL; j++) {
             const int cell = lane + 64 * j;
@@ -1241,7 +1257,9 @@ __device__ __forceinline__ void pipe_item(
             const int br = cl / (2 * HF), bc = cl - br * (2 * HF);
             afy[j] = HF + br;
             afx[j] = bc < HF ? bc : TX + bc;
-            acol[j] = (colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j])) * CB;
+            const unsigned ac = colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j]);
+            acol[j] = ac * CB;
+            aref[j] = uref_of(ac);
         }
         AV ar[AD][XPL][NA];
         auto aload = [&](int q, int s) __attribute__((always_inline)) {
@@ -1254,8 +1272,8 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
         for (int j = 0; j < AD; j++) aload(zs - 1 + j, j);
         Msg<R> mq[3];
-        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0));
-        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1));
+        mq[0] = msg_of(ha[0].x, ha[0].y, hb[0].x, hb[0].y, zs - 1, hfl(0), hur);
+        mq[1] = msg_of(ha[1].x, ha[1].y, hb[1].x, hb[1].y, zs, hfl(1), hur);
         mq[2] = zero_msg<R>();
         hload(zs - 1 + PD, 0);
         hload(zs + PD, 1);
@@ -1264,7 +1282,7 @@ __device__ __forceinline__ void pipe_item(
             const int a = zs + t;
             {
                 const int s1 = (k + 2) % PD;
-                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1));
+                mq[(k + 2) % 3] = msg_of(ha[s1].x, ha[s1].y, hb[s1].x, hb[s1].y, a + 1, hfl(s1), hur);
                 // messages past plane ze and arms past ze-1 feed only
                 // results past the chunk: re-read those planes instead
                 hload(a + 1 + PD < ze ? a + 1 + PD : ze, s1);
@@ -1274,7 +1292,7 @@ __device__ __forceinline__ void pipe_item(
 #pragma unroll
                 for (int j = 0; j < XPL; j++) {
                     if (!aw[j]) continue;
-                    fd[(a - 1) & 1][afy[j]][afx[j]] = arm_cell(ar[sa][j]);
+                    fd[(a - 1) & 1][afy[j]][afx[j]] = arm_cell(ar[sa][j], aref[j]);
                 }
             }
             aload(a + 2 < ze - 1 ? a + 2 : ze - 1, k % AD);
