@@ -706,6 +706,14 @@ __device__ __forceinline__ void pipe_item(
                                  : ((w << 13) & 0x0fffe000u) | ((w << 16) & 0x80000000u);
         return __builtin_bit_cast(float, bits) * 0x1p112f;
     };
+    // the same plus `add` in one fused multiply-add (the product is exact,
+    // so the result is the sum rounded once, as h2f(w) + add)
+    auto h2f_add = [](uint32_t w, auto hi, float add) __attribute__((always_inline)) -> float {
+        constexpr bool HI = decltype(hi)::value;
+        const uint32_t bits = HI ? ((w >> 3) & 0x0fffe000u) | (w & 0x80000000u)
+                                 : ((w << 13) & 0x0fffe000u) | ((w << 16) & 0x80000000u);
+        return __builtin_fmaf(__builtin_bit_cast(float, bits), 0x1p112f, add);
+    };
     using LO_ = std::integral_constant<bool, false>;
     using HI_ = std::integral_constant<bool, true>;
     // A corrector arm cell (ux, uy, uz) as the loads bring it: three
@@ -722,6 +730,12 @@ __device__ __forceinline__ void pipe_item(
             v[2] = bload<S>(rW, cb, so);
         }
     };
+    // RAWZ: the corrector tile and the decoded queue hold uz's departure
+    // from the reference, and the interior adds the references of the arms
+    // (constant along z) in one term.  For BF16 storage the decode has no
+    // multiply to fuse the reference into, and the adds on the halo waves'
+    // arm path cost 5% of the step; FP16 storage fuses them (h2f_add).
+    constexpr bool RAWZ = MODE == FSLBM_BF16S;
     // streamwise reference of column c (16-bit modes; 0 in FP32)
     auto uref_of = [&](unsigned c) __attribute__((always_inline)) -> R {
         if constexpr (PR) return P.q.uref ? P.q.uref[c] : R(0);
@@ -733,11 +747,12 @@ __device__ __forceinline__ void pipe_item(
         if constexpr (MODE == FSLBM_F16S) {
             cc.x = h2f(__builtin_bit_cast(uint32_t, v[0]), HI_{});
             cc.y = h2f(__builtin_bit_cast(uint32_t, v[1]), LO_{});
-            cc.z = h2f(__builtin_bit_cast(uint32_t, v[1]), HI_{}) + ur;
+            cc.z = h2f_add(__builtin_bit_cast(uint32_t, v[1]), HI_{}, ur);
         } else if constexpr (PR) {
             cc.x = ST::ld_u(v[0].y, P.q);
             cc.y = ST::ld_u(v[1].x, P.q);
-            cc.z = ST::ld_u(v[1].y, P.q) + ur;
+            cc.z = ST::ld_u(v[1].y, P.q);  // BF16: the departure (RAWZ)
+            (void)ur;
         } else {
             cc.x = ST::ld_u(v[0], P.q);
             cc.y = ST::ld_u(v[1], P.q);
@@ -777,7 +792,7 @@ __device__ __forceinline__ void pipe_item(
             auto w = [](S v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); };
             const R pp = fl ? ((P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(w(rp), LO_{})) + jump
                             : R(0);
-            return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f(w(rw), LO_{}) + ur);
+            return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f_add(w(rw), LO_{}, ur));
         } else {
             const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
             R uz = ST::ld_u(rw, P.q);
@@ -862,6 +877,22 @@ __device__ __forceinline__ void pipe_item(
             return M3 ? fluid && !(code & WC_WALL) : fluid;
         };
         const R urc = uref_of(col);  // the column's streamwise reference
+        // The stored uz' = uz - urc is folded into the update's first sum:
+        // nz_ = uz ku + S1 ks1 (+ S2 ks2) + ... - urc.  With RAWZ, uz and the
+        // arm sums S1, S2 are sums of departures, and the references they
+        // leave out (urc ku, and ks1, ks2 times the arms' references: four
+        // in-plane columns at distance d1 (2 d1) and twice urc for the z
+        // arms) join the constant.
+        R czr = -urc;
+        if constexpr (RAWZ) {
+            auto arms = [&](int d) {
+                return (uref_of(colx(x + d) + nx * coly(y)) + uref_of(colx(x - d) + nx * coly(y))) +
+                       (uref_of(colx(x) + nx * coly(y + d)) + uref_of(colx(x) + nx * coly(y - d))) +
+                       R(2) * urc;
+            };
+            czr = urc * P.ku - urc + arms(D1) * P.ks1;
+            if constexpr (FD2) czr += arms(2 * D1) * P.ks2;
+        }
         // Lanes that own no fluid cell store 0 to their (wall) column: in
         // a walled box an outside lane's clamped column is a wall.  In a
         // periodic box an outside lane computes a wrapped copy of an inside
@@ -912,19 +943,19 @@ __device__ __forceinline__ void pipe_item(
                 dq[d][0] = (P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(wa, LO_{});
                 dq[d][1] = h2f(wa, HI_{});
                 dq[d][2] = h2f(wb, LO_{});
-                dq[d][3] = h2f(wb, HI_{}) + urc;
+                dq[d][3] = h2f_add(wb, HI_{}, urc);
             } else {
                 dq[d][0] = ST::ld_p(qa[s].x, zp, P.q);
                 dq[d][1] = ST::ld_u(qa[s].y, P.q);
                 dq[d][2] = ST::ld_u(qb[s].x, P.q);
-                dq[d][3] = ST::ld_u(qb[s].y, P.q) + urc;
+                dq[d][3] = RAWZ ? ST::ld_u(qb[s].y, P.q) : ST::ld_u(qb[s].y, P.q) + urc;
             }
         };
         auto msg_dec = [&](int d, int zl, bool fl) __attribute__((always_inline)) -> Msg<R> {
             R jump;
             (void)phys(zl, jump);
             const R pp = fl ? dq[d][0] + jump : R(0);
-            return make_msg<R>(pp, dq[d][1], dq[d][2], dq[d][3]);
+            return make_msg<R>(pp, dq[d][1], dq[d][2], RAWZ ? dq[d][3] + urc : dq[d][3]);
         };
         bool flm = fluid, fl0 = fluid;  // planes zs-1 and zs
         if constexpr (M3) {
@@ -1055,7 +1086,8 @@ __device__ __forceinline__ void pipe_item(
                 // u' = tb (u*/216 + wall + g (c0 u + c1 S1 + c2 S2)) + (1 - tb) u
                 R nx_ = ux * P.ku + s1x * P.ks1;
                 R ny_ = uy * P.ku + s1y * P.ks1;
-                R nz_ = uz * P.ku + s1z * P.ks1;
+                // (the 16-bit modes store uz - uref: folded into this sum)
+                R nz_ = PR ? uz * P.ku + (s1z * P.ks1 + czr) : uz * P.ku + s1z * P.ks1;
                 if constexpr (FD2) {  // order-4 arms (blend != 0)
                     const FdCell<R> xp2 = fd[cb][cy_][cx_ + 2 * D1], xm2 = fd[cb][cy_][cx_ - 2 * D1];
                     const FdCell<R> yp2 = fd[cb][cy_ + 2 * D1][cx_], ym2 = fd[cb][cy_ - 2 * D1][cx_];
@@ -1081,7 +1113,7 @@ __device__ __forceinline__ void pipe_item(
                 // the loop's single scheduling region.
                 if constexpr (PR) {  // two packed pairs, one store each
                     const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.a, nx_, rw.a << 16);
-                    const uint32_t B = ST::sr2(ny_, rw.b, nz_ - urc, rw.b << 16);
+                    const uint32_t B = ST::sr2(ny_, rw.b, nz_, rw.b << 16);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? A : 0u, oP, sb_, so, 0);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? B : 0u, oV, sb_, so, 0);
                 } else {
