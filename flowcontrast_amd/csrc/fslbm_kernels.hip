@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256) void k_step_plain(
     }
 
     const R g = k.dtddy * k.m;
-    const SrWords rw = ST::SR ? sr_words((uint32_t)i, k.skey) : SrWords{0u, 0u};
+    const SrWords rw = ST::SR ? sr_words(sr_salt((uint32_t)col), sr_plane(k.skey, (uint32_t)z))
+                              : SrWords{0u, 0u};
     out[L::at(0, i, N)] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
     out[L::at(1, i, N)] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
     out[L::at(2, i, N)] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);

@@ -49,13 +49,39 @@ __host__ __device__ inline uint32_t sr_key(uint32_t seed)
 // Random words of one cell in one step.  The SR conversions take their
 // random bits from the top of the word, so a and a << 16 (b and b << 16)
 // give the four fields disjoint bits.
+//
+// A cell's words combine a per-column salt (a hash of the column index
+// x + nx*y, computed once per column) with two per-plane keys (hashes of
+// the step key and the plane index z, the same for every column of the
+// plane): word = (salt ^ key) * M, then its high half folded into its
+// low half.  For a fixed cell the keys are fresh hashes in every step, so
+// each word is uniform and independent from step to step (stochastic
+// rounding stays unbiased); columns have unrelated salts and planes
+// unrelated keys, so neighbouring cells along x, y and z draw unrelated
+// words (tests/test_rounding_words.py measures both).  In the kernels the
+// plane keys are uniform across a wave and computed on the scalar unit,
+// leaving one vector multiply per word (against two per word for a full
+// hash of the cell index).
 struct SrWords {
     uint32_t a, b;
 };
-__host__ __device__ inline SrWords sr_words(uint32_t cell, uint32_t key)
+struct SrPlane {
+    uint32_t a, b;
+};
+__host__ __device__ inline uint32_t sr_salt(uint32_t col)
 {
-    const uint32_t a = mix32(cell ^ key);
-    return {a, mix32(a + 0x9e3779b9u)};
+    return mix32(col + 0x2545f491u);
+}
+__host__ __device__ inline SrPlane sr_plane(uint32_t key, uint32_t z)
+{
+    const uint32_t a = mix32(key ^ (z * 0x9e3779b9u));
+    return {a, mix32(a + 0x632be5abu)};
+}
+__host__ __device__ inline SrWords sr_words(uint32_t salt, SrPlane k)
+{
+    const uint32_t a = (salt ^ k.a) * 0x846ca68bu;
+    const uint32_t b = (salt ^ k.b) * 0x7feb352du;
+    return {a ^ (a >> 16), b ^ (b >> 16)};
 }
 
 template <int MODE>
@@ -140,7 +166,9 @@ struct Store<FSLBM_F16S> {
     __device__ static uint32_t sr2(float lo, uint32_t rlo, float hi, uint32_t rhi)
     {
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-        h2 t = {0, 0};
+        // both halves are written below: the initial bits are any value
+        // at hand (no register needs clearing first)
+        h2 t = __builtin_bit_cast(h2, lo);
         t = __builtin_amdgcn_cvt_sr_f16_f32(t, lo, rlo, false);
         t = __builtin_amdgcn_cvt_sr_f16_f32(t, hi, rhi, true);
         return __builtin_bit_cast(uint32_t, t);
@@ -185,7 +213,7 @@ struct Store<FSLBM_BF16S> {
     __device__ static uint32_t sr2(float lo, uint32_t rlo, float hi, uint32_t rhi)
     {
         typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-        b2 t = {0, 0};
+        b2 t = __builtin_bit_cast(b2, lo);  // both halves are written below
         t = __builtin_amdgcn_cvt_sr_bf16_f32(t, lo, rlo, false);
         t = __builtin_amdgcn_cvt_sr_bf16_f32(t, hi, rhi, true);
         return __builtin_bit_cast(uint32_t, t);

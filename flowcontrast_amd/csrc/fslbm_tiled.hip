@@ -694,25 +694,36 @@ __device__ __forceinline__ void pipe_item(
         return v;
     };
     // Exact half-to-float of the low (HI false) or high half of a 32-bit
-    // word by integer operations: the half's sign, exponent and mantissa
-    // moved into place read as a float, times 2^112 (the exponent bias
+    // word by integer operations: the half moved to the top of a word and
+    // shifted right by 3 with sign extension puts its exponent and
+    // mantissa in place for a float and its sign in bits 31-28; clearing
+    // bits 30-28 (and, for the high half, the low half's bits) leaves a
+    // float of the half's value times 2^-112 (the exponent bias
     // difference; exact for normal and subnormal halves, as float
-    // denormals are preserved).  The conversion instructions
-    // (v_cvt_f32_f16, v_fma_mix_f32) need hazard padding and cost more
-    // than these operations: the FP16-storage step measured 8% faster.
-    auto h2f = [](uint32_t w, auto hi) __attribute__((always_inline)) -> float {
+    // denormals are preserved).  Two operations for the high half, three
+    // for the low one.  The conversion instructions (v_cvt_f32_f16,
+    // v_fma_mix_f32) need hazard padding and cost more than these
+    // operations: the FP16-storage step measured 8% faster without them.
+    auto h2bits = [](uint32_t w, auto hi) __attribute__((always_inline)) -> float {
         constexpr bool HI = decltype(hi)::value;
-        const uint32_t bits = HI ? ((w >> 3) & 0x0fffe000u) | (w & 0x80000000u)
-                                 : ((w << 13) & 0x0fffe000u) | ((w << 16) & 0x80000000u);
-        return __builtin_bit_cast(float, bits) * 0x1p112f;
+        const uint32_t bits = HI ? (uint32_t)((int32_t)w >> 3) & 0x8fffe000u
+                                 : (uint32_t)((int32_t)(w << 16) >> 3) & 0x8fffffffu;
+        return __builtin_bit_cast(float, bits);
+    };
+    auto h2f = [&](uint32_t w, auto hi) __attribute__((always_inline)) -> float {
+        return h2bits(w, hi) * 0x1p112f;
     };
     // the same plus `add` in one fused multiply-add (the product is exact,
     // so the result is the sum rounded once, as h2f(w) + add)
-    auto h2f_add = [](uint32_t w, auto hi, float add) __attribute__((always_inline)) -> float {
-        constexpr bool HI = decltype(hi)::value;
-        const uint32_t bits = HI ? ((w >> 3) & 0x0fffe000u) | (w & 0x80000000u)
-                                 : ((w << 13) & 0x0fffe000u) | ((w << 16) & 0x80000000u);
-        return __builtin_fmaf(__builtin_bit_cast(float, bits), 0x1p112f, add);
+    auto h2f_add = [&](uint32_t w, auto hi, float add) __attribute__((always_inline)) -> float {
+        return __builtin_fmaf(h2bits(w, hi), 0x1p112f, add);
+    };
+    // a stored FP16 pressure, base + sp * value: the 2^112 of the decode
+    // joins the power-of-two scale sp (an exact product), so the value is
+    // one fused multiply-add of the decoded bits, rounded once as before
+    const float sp112 = P.q.sp * 0x1p112f;
+    auto h2p = [&](uint32_t w, float base) __attribute__((always_inline)) -> float {
+        return __builtin_fmaf(sp112, h2bits(w, std::false_type{}), base);
     };
     using LO_ = std::integral_constant<bool, false>;
     using HI_ = std::integral_constant<bool, true>;
@@ -790,8 +801,7 @@ __device__ __forceinline__ void pipe_item(
         const int zp = phys(zl, jump);
         if constexpr (MODE == FSLBM_F16S) {
             auto w = [](S v) { return (uint32_t)__builtin_bit_cast(uint16_t, v); };
-            const R pp = fl ? ((P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(w(rp), LO_{})) + jump
-                            : R(0);
+            const R pp = fl ? h2p(w(rp), P.q.pbase + P.q.pslope * (float)zp) + jump : R(0);
             return make_msg<R>(pp, h2f(w(ru), LO_{}), h2f(w(rv), LO_{}), h2f_add(w(rw), LO_{}, ur));
         } else {
             const R pp = fl ? ST::ld_p(rp, zp, P.q) + jump : R(0);
@@ -903,6 +913,8 @@ __device__ __forceinline__ void pipe_item(
         // stays below 2^32: a periodic box's fields are below 2^31 bytes,
         // tiled_supported.)
         const bool store = PERIODIC ? true : (inside && fluid);
+        // stochastic-rounding salt of the column (16-bit modes)
+        const uint32_t salt = ST::SR ? sr_salt(col) : 0u;
         const unsigned cb_ = col * CB;
         const unsigned sb_ = (PERIODIC && !inside) ? 0x80000000u : cb_;
         const __amdgpu_buffer_rsrc_t oP = mkbuf(out, fbytes), oV = mkbuf(out + 2ll * N, fbytes);
@@ -940,7 +952,7 @@ __device__ __forceinline__ void pipe_item(
             if constexpr (MODE == FSLBM_F16S) {
                 const uint32_t wa = __builtin_bit_cast(uint32_t, qa[s]),
                                wb = __builtin_bit_cast(uint32_t, qb[s]);
-                dq[d][0] = (P.q.pbase + P.q.pslope * (float)zp) + P.q.sp * h2f(wa, LO_{});
+                dq[d][0] = h2p(wa, P.q.pbase + P.q.pslope * (float)zp);
                 dq[d][1] = h2f(wa, HI_{});
                 dq[d][2] = h2f(wb, LO_{});
                 dq[d][3] = h2f_add(wb, HI_{}, urc);
@@ -1104,7 +1116,7 @@ __device__ __forceinline__ void pipe_item(
                 const R np_ = (Ps * inv216 + w0) - P.gauge;
                 const unsigned so = poff(c);
                 const S z0 = S(0);
-                const SrWords rw = ST::SR ? sr_words((uint32_t)c * (uint32_t)nxy + col, skey)
+                const SrWords rw = ST::SR ? sr_words(salt, sr_plane(skey, (uint32_t)c))
                                           : SrWords{0u, 0u};
                 // Encoded on every lane, then selected: written as
                 // `store ? encode(..) : 0`, the stochastic-rounding
