@@ -182,14 +182,14 @@ __global__ __launch_bounds__(256) void k_step_plain(
 
     const R g = k.dtddy * k.m;
     const SrWords rw = ST::SR ? sr_words(sr_salt((uint32_t)col), sr_plane(k.skey, (uint32_t)z))
-                              : SrWords{0u, 0u};
-    out[L::at(0, i, N)] = ST::st_p_sr(rt - k.gauge, z, q, rw.a);
-    out[L::at(1, i, N)] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.a << 16);
-    out[L::at(2, i, N)] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.b);
+                              : SrWords{{0u, 0u, 0u, 0u}};
+    out[L::at(0, i, N)] = ST::st_p_sr(rt - k.gauge, z, q, rw.r[0]);
+    out[L::at(1, i, N)] = ST::st_u_sr((ax + g * Lx) * k.c + us * k.cc, q, rw.r[1]);
+    out[L::at(2, i, N)] = ST::st_u_sr((ay + g * Ly) * k.c + vs * k.cc, q, rw.r[2]);
     R wn = (az + g * Lz) * k.c + ws * k.cc;
     if constexpr (ST::SR)
         if (q.uref) wn -= q.uref[col];
-    out[L::at(3, i, N)] = ST::st_u_sr(wn, q, rw.b << 16);
+    out[L::at(3, i, N)] = ST::st_u_sr(wn, q, rw.r[3]);
 }
 
 template <int MODE>

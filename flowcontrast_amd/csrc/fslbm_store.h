@@ -46,24 +46,29 @@ __host__ __device__ inline uint32_t sr_key(uint32_t seed)
     return mix32(seed * 0x9e3779b9u + 0x632be5abu);
 }
 
-// Random words of one cell in one step.  The SR conversions take their
-// random bits from the top of the word, so a and a << 16 (b and b << 16)
-// give the four fields disjoint bits.
+// Random operands of the stochastic rounding of one cell in one step,
+// one per field (p, ux, uy, uz).  The SR conversions take their random
+// bits from the top of the operand (13 bits for binary16, 16 for
+// bfloat16).
 //
-// A cell's words combine a per-column salt (a hash of the column index
-// x + nx*y, computed once per column) with two per-plane keys (hashes of
-// the step key and the plane index z, the same for every column of the
-// plane): word = (salt ^ key) * M, then its high half folded into its
-// low half.  For a fixed cell the keys are fresh hashes in every step, so
-// each word is uniform and independent from step to step (stochastic
-// rounding stays unbiased); columns have unrelated salts and planes
-// unrelated keys, so neighbouring cells along x, y and z draw unrelated
-// words (tests/test_rounding_words.py measures both).  In the kernels the
-// plane keys are uniform across a wave and computed on the scalar unit,
-// leaving one vector multiply per word (against two per word for a full
-// hash of the cell index).
+// They come from two words per cell, each a per-column salt (a hash of the
+// column index x + nx*y, computed once per column) combined with a
+// per-plane key (a hash of the step key and the plane index z, the same
+// for every column of the plane): x = (salt ^ key) * M, then its high half
+// folded into its low half, w = x ^ (x >> 16).  Fields 0 and 2 draw the
+// top half of their word w, fields 1 and 3 its bottom half (w << 16).  A
+// word is uniform over 2^32 values, so its two halves are independent;
+// the fold makes the bottom half depend on every bit of the product (the
+// product's own low bits depend on the low bits of its operand only).  For a fixed cell the keys are fresh hashes in
+// every step, so each draw is uniform and independent from step to step
+// (stochastic rounding stays unbiased); columns have unrelated salts and
+// planes unrelated keys, so neighbouring cells along x, y and z draw
+// unrelated values (tests/test_rounding_words.py measures both).  In the
+// kernels the plane keys are uniform across a wave and computed on the
+// scalar unit, leaving one vector multiply per word (against two per word
+// for a full hash of the cell index).
 struct SrWords {
-    uint32_t a, b;
+    uint32_t r[4];
 };
 struct SrPlane {
     uint32_t a, b;
@@ -79,9 +84,11 @@ __host__ __device__ inline SrPlane sr_plane(uint32_t key, uint32_t z)
 }
 __host__ __device__ inline SrWords sr_words(uint32_t salt, SrPlane k)
 {
-    const uint32_t a = (salt ^ k.a) * 0x846ca68bu;
-    const uint32_t b = (salt ^ k.b) * 0x7feb352du;
-    return {a ^ (a >> 16), b ^ (b >> 16)};
+    uint32_t x = (salt ^ k.a) * 0x846ca68bu;
+    uint32_t y = (salt ^ k.b) * 0x7feb352du;
+    x ^= x >> 16;
+    y ^= y >> 16;
+    return {{x, x << 16, y, y << 16}};
 }
 
 template <int MODE>

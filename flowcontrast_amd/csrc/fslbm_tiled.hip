@@ -777,11 +777,14 @@ __device__ __forceinline__ void pipe_item(
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
 
+    // The physical plane of plane zl (z wraps) and the pressure jump of the
+    // wrap.  zl is the same on every lane: the jump is selected as bits,
+    // which keeps the selects on the scalar unit.
     auto phys = [&](int zl, R &jump) -> int {
-        jump = R(0);
-        if (zl < 0) { jump = P.pin; return zl + nz; }
-        if (zl >= nz) { jump = -P.pin; return zl - nz; }
-        return zl;
+        const uint32_t pb = __builtin_bit_cast(uint32_t, P.pin);
+        const uint32_t jb = zl < 0 ? pb : zl >= nz ? pb ^ 0x80000000u : 0u;
+        jump = __builtin_bit_cast(R, jb);
+        return zl < 0 ? zl + nz : zl >= nz ? zl - nz : zl;
     };
     auto colx = [&](int v) { return PERIODIC ? modp(v, nx) : clampi(v, nx); };
     auto coly = [&](int v) { return PERIODIC ? modp(v, ny) : clampi(v, ny); };
@@ -1118,22 +1121,22 @@ __device__ __forceinline__ void pipe_item(
                 const unsigned so = poff(c);
                 const S z0 = S(0);
                 const SrWords rw = ST::SR ? sr_words(salt, sr_plane(skey, (uint32_t)c))
-                                          : SrWords{0u, 0u};
+                                          : SrWords{{0u, 0u, 0u, 0u}};
                 // Encoded on every lane, then selected: written as
                 // `store ? encode(..) : 0`, the stochastic-rounding
                 // conversions (which the compiler does not speculate)
                 // became four exec-masked branches per plane that broke
                 // the loop's single scheduling region.
                 if constexpr (PR) {  // two packed pairs, one store each
-                    const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.a, nx_, rw.a << 16);
-                    const uint32_t B = ST::sr2(ny_, rw.b, nz_, rw.b << 16);
+                    const uint32_t A = ST::sr2(ST::enc_p(np_, c, P.q), rw.r[0], nx_, rw.r[1]);
+                    const uint32_t B = ST::sr2(ny_, rw.r[2], nz_, rw.r[3]);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? A : 0u, oP, sb_, so, 0);
                     __builtin_amdgcn_raw_buffer_store_b32(st_c ? B : 0u, oV, sb_, so, 0);
                 } else {
-                    const S ep = ST::st_p_sr(np_, c, P.q, rw.a);
-                    const S ex = ST::st_u_sr(nx_, P.q, rw.a << 16);
-                    const S ey = ST::st_u_sr(ny_, P.q, rw.b);
-                    const S ez = ST::st_u_sr(nz_, P.q, rw.b << 16);
+                    const S ep = ST::st_p_sr(np_, c, P.q, rw.r[0]);
+                    const S ex = ST::st_u_sr(nx_, P.q, rw.r[1]);
+                    const S ey = ST::st_u_sr(ny_, P.q, rw.r[2]);
+                    const S ez = ST::st_u_sr(nz_, P.q, rw.r[3]);
                     bstore<NTS>(st_c ? ep : z0, oP, sb_, so);
                     bstore<NTS>(st_c ? ex : z0, oU, sb_, so);
                     bstore<NTS>(st_c ? ey : z0, oV, sb_, so);

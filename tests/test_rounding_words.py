@@ -1,9 +1,9 @@
 """Statistics of the stochastic-rounding words of the 16-bit storage modes.
 
 The kernels round each stored 16-bit value stochastically with a random
-word per cell and step (fslbm_store.h: sr_salt, sr_plane, sr_words).  The
-conversion takes its random bits from the top of the word; the four fields
-of a cell use a, a << 16, b and b << 16.  Rounding is unbiased when, for
+operand per field, cell and step (fslbm_store.h: sr_salt, sr_plane,
+sr_words).  The conversion takes its random bits from the top of the
+operand (13 bits for binary16, 16 for bfloat16).  Rounding is unbiased when, for
 every cell, these 16-bit draws are uniform and independent from step to
 step; neighbouring cells and the four fields of one cell should draw
 unrelated values, so that rounding errors do not line up in space.
@@ -38,11 +38,9 @@ def draws():
                        capture_output=True)
     raw = subprocess.run([str(HARNESS), str(NX), str(NY), str(NZ), str(STEPS), "11"],
                          check=True, capture_output=True).stdout
-    w = np.frombuffer(raw, dtype="<u4").reshape(STEPS, NZ, NY, NX, 2)
-    a, b = w[..., 0], w[..., 1]
-    # the 16 random bits each field's conversion sees, as uniforms in [0, 1)
-    fields = [a >> 16, (a << 16) >> 16 & 0xffff, b >> 16, (b << 16) >> 16 & 0xffff]
-    return np.stack([(f & 0xffff).astype(np.float64) / 65536.0 for f in fields])
+    w = np.frombuffer(raw, dtype="<u4").reshape(STEPS, NZ, NY, NX, 4)
+    # the top 16 bits each field's conversion sees, as uniforms in [0, 1)
+    return np.stack([(w[..., f] >> 16).astype(np.float64) / 65536.0 for f in range(4)])
 
 
 def corr(u, v):
