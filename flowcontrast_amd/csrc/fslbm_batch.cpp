@@ -436,21 +436,28 @@ size_t domain_bytes(int precision, int nx, int ny, int nz, bool masked)
 
 namespace fslbm {
 
-// Validate and encode a domain on the host (no device work; safe while
-// the batch's launches run).  The mask (NULL: periodic box) is one z plane.
-int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
-                  int ny, int nz, const double *const src[4], PreparedDomain &pd)
+// Check a domain against what a batch of this precision and d1 runs (no
+// batch state is touched, so any thread may call it).  Returns FSLBM_OK, or
+// an error with the reason in `why`.
+int domain_check(int precision, int d1, const fslbm_params *prm, const uint8_t *mask, int nx,
+                 int ny, int nz, bool have_state, std::string &why)
 {
-    if (!b) return FSLBM_EINVAL;
-    if (!prm || !src[0] || !src[1] || !src[2] || !src[3])
-        return bfail(b, FSLBM_EINVAL, "NULL argument");
-    if (nx < 1 || ny < 1 || nz < 1) return bfail(b, FSLBM_EINVAL, "bad box");
+    auto bad = [&](const char *m) {
+        why = m;
+        return FSLBM_EINVAL;
+    };
+    if (!prm || !have_state) return bad("NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return bad("bad box");
     const long long N = (long long)nx * ny * nz, nxy = (long long)nx * ny;
-    if (N >= (1ll << 31)) return bfail(b, FSLBM_EINVAL, "domain too large (N >= 2^31 cells)");
-    if (prm->d1 != b->d1) return bfail(b, FSLBM_EINVAL, "domain d1 %d differs from the batch's %d", prm->d1, b->d1);
+    if (N >= (1ll << 31)) return bad("domain too large (N >= 2^31 cells)");
+    if (prm->d1 != d1) {
+        why = "domain d1 " + std::to_string(prm->d1) + " differs from the batch's " +
+              std::to_string(d1);
+        return FSLBM_EINVAL;
+    }
     if (!std::isfinite(prm->tblend) || !std::isfinite(prm->pin) || !std::isfinite(prm->nu) ||
         !std::isfinite(prm->blend))
-        return bfail(b, FSLBM_EINVAL, "non-finite run parameter");
+        return bad("non-finite run parameter");
     Geo g;
     g.nx = nx;
     g.ny = ny;
@@ -458,42 +465,70 @@ int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, 
     g.N = N;
     g.periodic = mask == nullptr;
     if (g.periodic && (2 * prm->d1 > nx || 2 * prm->d1 > ny))
-        return bfail(b, FSLBM_EINVAL, "periodic box narrower than the corrector reach");
+        return bad("periodic box narrower than the corrector reach");
     if (mask) {
         for (long long i = 0; i < nxy; i++) {
             const uint8_t m = mask[i];
-            if (m != 0 && m != 125) return bfail(b, FSLBM_EINVAL, "mask codes must be 0 or 125");
+            if (m != 0 && m != 125) return bad("mask codes must be 0 or 125");
             const int x = (int)(i % nx), y = (int)(i / nx);
             if (m == 0 && (x == 0 || y == 0 || x == nx - 1 || y == ny - 1))
-                return bfail(b, FSLBM_EINVAL, "fluid cell on the x/y face; a wall frame is required");
+                return bad("fluid cell on the x/y face; a wall frame is required");
         }
     }
-    if (!tiled_supported(g, b->precision, prm->d1, true))
-        return bfail(b, FSLBM_EINVAL, "domain not supported by the tiled kernel (nz >= 16 required)");
+    if (!tiled_supported(g, precision, prm->d1, true))
+        return bad("domain not supported by the tiled kernel (nz >= 16 required)");
+    return FSLBM_OK;
+}
+
+// Encode a checked domain on the host (no device work, no batch state:
+// any thread may call it, also while the batch's launches run).
+void domain_encode(int precision, const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
+                   int nz, const double *const src[4], PreparedDomain &pd)
+{
+    const long long N = (long long)nx * ny * nz, nxy = (long long)nx * ny;
+    Geo g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.N = N;
+    g.periodic = mask == nullptr;
     pd = PreparedDomain();
     pd.g = g;
     pd.prm = *prm;
-    pd.precision = b->precision;
-    pd.bytes = domain_bytes(b->precision, nx, ny, nz, mask != nullptr);
+    pd.precision = precision;
+    pd.bytes = domain_bytes(precision, nx, ny, nz, mask != nullptr);
     if (mask) pd.mask2d.assign(mask, mask + nxy);
     HostMask hm;
     hm.m = mask ? pd.mask2d.data() : nullptr;
     hm.nxy = nxy;
     hm.plane = true;
-    if (b->precision != FSLBM_F32) {
+    if (precision != FSLBM_F32) {
         choose_quant(g, hm, prm->pin, src, pd.q);
         pd.uref = choose_uref(g, hm, src);
     }
-    pd.host.resize(4 * (size_t)N * storage_bytes(b->precision));
-    encode_state(g, hm, b->precision, pd.q, src, pd.host.data(),
+    pd.host.resize(4 * (size_t)N * storage_bytes(precision));
+    encode_state(g, hm, precision, pd.q, src, pd.host.data(),
                  pd.uref.empty() ? nullptr : pd.uref.data());
     if (mask) {
         pd.omega = wall_weights<float>(nx, ny, pd.mask2d.data());
         // planned with the rows of the domain's kernel class (batch_fill_dom
         // and the class launch take the same)
-        pd.tiles = active_tiles(g, plan_rows(b->precision, false, prm->blend != 0.0),
+        pd.tiles = active_tiles(g, plan_rows(precision, false, prm->blend != 0.0),
                                 pd.mask2d.data());
     }
+}
+
+// Validate and encode a domain on the host (no device work; safe while
+// the batch's launches run).  The mask (NULL: periodic box) is one z plane.
+int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
+                  int ny, int nz, const double *const src[4], PreparedDomain &pd)
+{
+    if (!b) return FSLBM_EINVAL;
+    std::string why;
+    const bool have = src[0] && src[1] && src[2] && src[3];
+    const int rc = domain_check(b->precision, b->d1, prm, mask, nx, ny, nz, have, why);
+    if (rc) return bfail(b, rc, "%s", why.c_str());
+    domain_encode(b->precision, prm, mask, nx, ny, nz, src, pd);
     return FSLBM_OK;
 }
 

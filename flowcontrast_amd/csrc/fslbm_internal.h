@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/fslbm.h"
@@ -163,6 +164,13 @@ struct PreparedDomain;  // fslbm_codec.h
 // and admit it; batch_add_plane does both.
 int batch_prepare(fslbm_batch *b, const fslbm_params *prm, const uint8_t *mask, int nx,
                   int ny, int nz, const double *const src[4], PreparedDomain &pd);
+// The two halves of batch_prepare, which touch no batch state (any thread
+// may call them): the check (FSLBM_OK, or an error with the reason in why)
+// and the encoding of a checked domain.
+int domain_check(int precision, int d1, const fslbm_params *prm, const uint8_t *mask, int nx,
+                 int ny, int nz, bool have_state, std::string &why);
+void domain_encode(int precision, const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
+                   int nz, const double *const src[4], PreparedDomain &pd);
 // attach uploads on stream s (nullptr: the batch's) and waits for it if
 // sync; otherwise the caller keeps pd's host buffers until it has.
 int batch_attach(fslbm_batch *b, PreparedDomain &pd, hipStream_t s, bool sync);

@@ -7,16 +7,22 @@
 //      running them one after another);
 //   2. while those run, copies out the results of the requests that
 //      finished in the previous pump (they stay resident, no longer
-//      stepped), encodes queued requests on the host in arrival order while
-//      the HBM budget holds them, and uploads them, all on a copy stream;
+//      stepped) and uploads the requests the encoder has made ready, all
+//      on a copy stream;
 //   3. waits for both;
 //   4. frees the copied-out requests and hands them back, and lets the
 //      uploaded ones step from the next pump on;
 //   5. holds the requests whose step budget is now spent.
 // The pump's step count for a mode is the smallest remaining budget of
 // its stepping requests, capped by the quantum, so no request runs past
-// its budget.  Device memory of a request counts against the budget from
-// its upload to its hand-back.
+// its budget.
+//
+// Requests are checked when they are submitted (an invalid one fails
+// alone, at once) and encoded into their stored form by a host thread of
+// the server's own, in arrival order while the HBM budget holds them, so
+// that encoding overlaps the steps of the resident requests instead of
+// waiting for the next pump.  A request's memory counts against the budget
+// from the start of its encoding to its hand-back.
 //
 // Which resident requests step in a pump is the policy's choice:
 //   FSLBM_SERVE_SHARE (default): all of them (every request advances at
@@ -32,12 +38,16 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fslbm_codec.h"
@@ -60,14 +70,16 @@ int mode_index(int precision)
     }
 }
 
-enum class RState { Queued, Ready, Running, Held, Done, Failed };
+enum class RState { Queued, Encoding, Ready, Running, Held, Done, Failed };
 
 struct Request {
     fslbm_request r{};
     bool paused = false;  // resident and running, held in its batch this pump
     int waited = 0;       // consecutive pumps paused
     std::vector<uint8_t> mask2d;  // one z plane; empty for a periodic box
-    std::vector<double> st[4];    // initial state until admission
+    // initial state until admission (not value-initialised: filled by the
+    // copy at submission)
+    std::unique_ptr<double[]> st[4];
     int mode = 0;
     int slot = -1;
     long long left = 0;
@@ -88,14 +100,26 @@ struct fslbm_server {
     int quantum = 100;
     int policy = FSLBM_SERVE_SHARE;
     long long fill_cells = 0;  // 0: 32768 cells per compute unit
-    size_t budget = 0, used = 0;
+    size_t budget = 0;
     fslbm_batch_t batch[NMODES] = {nullptr, nullptr, nullptr};
-    std::deque<long long> queue;        // arrival order
-    std::vector<long long> ready;       // encoded, uploading in this pump
+    // Shared with the encoder thread, under mu: the request map's
+    // structure, the queue, the encoded list, the budget in use and the
+    // state of queued and encoding requests.  Everything else belongs to
+    // the caller's thread.
+    std::mutex mu;
+    std::condition_variable cv;        // queue, budget or stop changed (encoder waits)
+    std::condition_variable cv_ready;  // a request became ready (an idle pump waits)
+    bool stop = false;
+    int encoding = 0;  // requests the encoder is working on
+    size_t used = 0;
+    std::deque<long long> queue;        // checked, waiting to be encoded (arrival order)
+    std::vector<long long> ready;       // encoded, waiting for upload
+    std::vector<long long> enc_failed;  // encoding failed (host memory)
+    std::thread encoder;
+    int failed_at_submit = 0;           // reported by the next pump's count
     std::vector<long long> running;     // stepping, in admission order
     std::vector<long long> held;        // finished, result copied out next pump
     hipStream_t copy = nullptr;         // result copies and uploads
-    double last_steps_s = 0.0;          // wall time of the last pump's steps
     std::map<long long, Request> reqs;  // every request not yet dropped
     std::deque<long long> unreported;   // finished, not yet popped
     long long next_id = 0;
@@ -123,7 +147,7 @@ int sfail(fslbm_server *s, int code, const char *fmt, ...)
 
 void free_state(Request &q)
 {
-    for (auto &v : q.st) std::vector<double>().swap(v);
+    for (auto &v : q.st) v.reset();
 }
 
 int ensure_batch(fslbm_server *s, int mode)
@@ -150,44 +174,54 @@ void finish(fslbm_server *s, long long id, Request &q, RState st, int status, do
     s->unreported.push_back(id);
 }
 
-// Encode queued requests in arrival order while the budget holds them
-// (host work, run while the pump's steps execute); memory of held results
-// still counts, as they are freed only after this pump's copies.  The host
-// time spent is bounded by the last pump's step time (at least one
-// request), so the encoding stays hidden behind the steps and the GPU is
-// not left waiting on a long admission.  Returns the number of requests
-// that failed validation, or an error.
-int prepare_queued(fslbm_server *s)
+// The encoder thread: takes the queue's first request once the budget
+// holds it (arrival order, no overtaking), reserves its memory and encodes
+// it into its stored form without holding the lock, then lists it ready
+// for upload.  The encoding touches no batch or device state.
+void encoder_loop(fslbm_server *s)
 {
-    int failed = 0;
-    const double t0 = s->now();
-    const double budget_s = std::max(s->last_steps_s, 0.005);
-    int prepared = 0;
-    while (!s->queue.empty()) {
-        if (prepared > 0 && s->now() - t0 > budget_s) break;
+    static const int prec[NMODES] = {FSLBM_F32, FSLBM_F16S, FSLBM_BF16S};
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv.wait(lk, [&] {
+            return s->stop ||
+                   (!s->queue.empty() && s->used + s->reqs.at(s->queue.front()).bytes <= s->budget);
+        });
+        if (s->stop) return;
         const long long id = s->queue.front();
-        Request &q = s->reqs.at(id);
-        if (s->used + q.bytes > s->budget) break;  // FIFO: no overtaking
-        int rc = ensure_batch(s, q.mode);
-        if (rc) return rc;
-        const double *src[4] = {q.st[0].data(), q.st[1].data(), q.st[2].data(), q.st[3].data()};
-        rc = batch_prepare(s->batch[q.mode], &q.r.params,
-                           q.mask2d.empty() ? nullptr : q.mask2d.data(), q.r.nx, q.r.ny,
-                           q.r.nz, src, q.prep);
         s->queue.pop_front();
-        free_state(q);
-        if (rc < 0) {
-            // an invalid request fails alone; the loop goes on
-            finish(s, id, q, RState::Failed, rc, s->now());
-            failed++;
-            continue;
+        Request &q = s->reqs.at(id);  // map nodes stay put; the request is ours until ready
+        s->used += q.bytes;
+        q.state = RState::Encoding;
+        s->encoding++;
+        lk.unlock();
+        bool ok = true;
+        try {
+            const double *src[4] = {q.st[0].get(), q.st[1].get(), q.st[2].get(), q.st[3].get()};
+            domain_encode(prec[q.mode], &q.r.params, q.mask2d.empty() ? nullptr : q.mask2d.data(),
+                          q.r.nx, q.r.ny, q.r.nz, src, q.prep);
+        } catch (...) {  // host memory exhausted
+            q.prep = PreparedDomain();
+            ok = false;
         }
-        s->used += q.bytes;  // reserved until the request is handed back
-        q.state = RState::Ready;
-        s->ready.push_back(id);
-        prepared++;
+        free_state(q);
+        lk.lock();
+        s->encoding--;
+        if (ok) {
+            q.state = RState::Ready;
+            s->ready.push_back(id);
+        } else {
+            s->enc_failed.push_back(id);
+        }
+        s->cv_ready.notify_all();
     }
-    return failed;
+}
+
+// Whether the encoder has work in hand or can start some (under mu).
+bool encoder_busy(const fslbm_server *s)
+{
+    return s->encoding > 0 ||
+           (!s->queue.empty() && s->used + s->reqs.at(s->queue.front()).bytes <= s->budget);
 }
 
 long long cells_of(const Request &q) { return (long long)q.r.nx * q.r.ny * q.r.nz; }
@@ -265,6 +299,13 @@ int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *
     s->quantum = quantum;
     s->budget = hbm_budget;
     s->t0 = std::chrono::steady_clock::now();
+    try {
+        s->encoder = std::thread(encoder_loop, s);
+    } catch (...) {
+        (void)hipStreamDestroy(s->copy);
+        delete s;
+        return sfail(nullptr, FSLBM_ENOMEM, "encoder thread creation failed");
+    }
     *out = s;
     return FSLBM_OK;
 }
@@ -272,6 +313,12 @@ int fslbm_server_create(int d1, size_t hbm_budget, int quantum, fslbm_server_t *
 void fslbm_server_destroy(fslbm_server_t s)
 {
     if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->encoder.joinable()) s->encoder.join();
     if (s->copy) (void)hipStreamSynchronize(s->copy);
     for (auto &b : s->batch)
         if (b) fslbm_batch_destroy(b);
@@ -312,17 +359,45 @@ long long fslbm_server_submit(fslbm_server_t s, const fslbm_request *req, const 
     if ((size_t)bytes > s->budget)
         return sfail(s, FSLBM_EINVAL, "request needs %lld device bytes, more than the budget %zu",
                      bytes, s->budget);
-    const long long id = s->next_id++;
-    Request &q = s->reqs[id];
-    q.r = *req;
-    q.mode = mode;
-    q.left = req->nsteps;
-    q.bytes = (size_t)bytes;
-    if (mask) q.mask2d.assign(mask, mask + nxy);
-    const double *src[4] = {p, ux, uy, uz};
-    for (int f = 0; f < 4; f++) q.st[f].assign(src[f], src[f] + N);
-    q.t_submit = s->now();
-    s->queue.push_back(id);
+    // A request the batch kernels cannot run (too short for the tiled
+    // kernel, bad mask codes, ...) is accepted and fails alone at once: its
+    // completion carries the error.
+    std::string why;
+    const int check = domain_check(req->precision, s->d1, &req->params, mask, req->nx, req->ny,
+                                   req->nz, true, why);
+    Request fresh;
+    fresh.r = *req;
+    fresh.mode = mode;
+    fresh.left = req->nsteps;
+    fresh.bytes = (size_t)bytes;
+    fresh.t_submit = s->now();
+    if (check == FSLBM_OK) {
+        if (mask) fresh.mask2d.assign(mask, mask + nxy);
+        // the caller's arrays are copied (the caller may free them on
+        // return), on the codec threads: a large request's state is
+        // gigabytes
+        const double *src[4] = {p, ux, uy, uz};
+        for (int f = 0; f < 4; f++) fresh.st[f].reset(new double[(size_t)N]);
+        par_blocks(N, CODEC_BLOCK, [&](long long lo, long long hi, long long) {
+            for (int f = 0; f < 4; f++)
+                std::memcpy(fresh.st[f].get() + lo, src[f] + lo,
+                            sizeof(double) * (size_t)(hi - lo));
+        });
+    }
+    long long id;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        id = s->next_id++;
+        Request &q = s->reqs[id];
+        q = std::move(fresh);
+        if (check == FSLBM_OK) {
+            s->queue.push_back(id);
+        } else {
+            finish(s, id, q, RState::Failed, check, q.t_submit);
+            s->failed_at_submit++;
+        }
+    }
+    if (check == FSLBM_OK) s->cv.notify_one();
     return id;
 }
 
@@ -341,7 +416,6 @@ int fslbm_server_pump(fslbm_server_t s)
         m = m == 0 ? q.left : std::min(m, q.left);
     }
     int rc;
-    const double t_launch = s->now();
     for (int k = 0; k < NMODES; k++) {
         qs[k] = std::min<long long>(qs[k], s->quantum);
         if (qs[k] > 0) {
@@ -349,23 +423,46 @@ int fslbm_server_pump(fslbm_server_t s)
             if (rc) return sfail(s, rc, "batch step: %s", fslbm_batch_last_error(s->batch[k]));
         }
     }
-    // 2. meanwhile: copy out last pump's results, encode and upload queued
-    // requests (copy stream)
+    // 2. meanwhile: copy out last pump's results and upload the requests
+    // the encoder has made ready (copy stream)
     for (long long id : s->held) {
         Request &q = s->reqs.at(id);
         rc = batch_copy_out(s->batch[q.mode], q.slot, q.res, s->copy);
         if (rc) return sfail(s, rc, "result copy: %s", fslbm_batch_last_error(s->batch[q.mode]));
     }
-    rc = prepare_queued(s);
-    if (rc < 0) return rc;
-    int finished = rc;
-    std::vector<long long> uploaded;
-    for (long long id : s->ready) {
+    bool stepping_now = false;
+    for (int k = 0; k < NMODES; k++) stepping_now |= qs[k] > 0;
+    int finished = 0;
+    std::vector<long long> ready, enc_failed;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        // A pump with no steps to run waits for the encoder's next request
+        // (the device has nothing else to do); otherwise it takes what is
+        // ready and leaves the rest to later pumps.
+        if (!stepping_now && s->held.empty())
+            s->cv_ready.wait(lk, [&] {
+                return !s->ready.empty() || !s->enc_failed.empty() || !encoder_busy(s);
+            });
+        ready.swap(s->ready);
+        enc_failed.swap(s->enc_failed);
+        finished += s->failed_at_submit;
+        s->failed_at_submit = 0;
+    }
+    size_t freed = 0;  // budget released in this pump
+    for (long long id : enc_failed) {
         Request &q = s->reqs.at(id);
-        rc = batch_attach(s->batch[q.mode], q.prep, s->copy, false);
+        freed += q.bytes;
+        finish(s, id, q, RState::Failed, FSLBM_ENOMEM, s->now());
+        finished++;
+    }
+    std::vector<long long> uploaded;
+    for (long long id : ready) {
+        Request &q = s->reqs.at(id);
+        rc = ensure_batch(s, q.mode);
+        if (rc == FSLBM_OK) rc = batch_attach(s->batch[q.mode], q.prep, s->copy, false);
         if (rc < 0) {
             q.prep = PreparedDomain();
-            s->used -= q.bytes;
+            freed += q.bytes;
             finish(s, id, q, RState::Failed, rc, s->now());
             finished++;
             continue;
@@ -373,7 +470,6 @@ int fslbm_server_pump(fslbm_server_t s)
         q.slot = rc;
         uploaded.push_back(id);
     }
-    s->ready.clear();
     // 3. wait for the steps and the copies
     for (int k = 0; k < NMODES; k++)
         if (qs[k] > 0) {
@@ -383,20 +479,24 @@ int fslbm_server_pump(fslbm_server_t s)
     if (hipStreamSynchronize(s->copy) != hipSuccess)
         return sfail(s, FSLBM_EHIP, "copy stream failed");
     const double t = s->now();
-    bool stepped = false;
-    for (int k = 0; k < NMODES; k++) stepped |= qs[k] > 0;
-    if (stepped) s->last_steps_s = t - t_launch;
     // 4. hand back the copied-out results, freeing their memory
     for (long long id : s->held) {
         Request &q = s->reqs.at(id);
         rc = fslbm_batch_remove(s->batch[q.mode], q.slot);
         if (rc) return sfail(s, rc, "remove: %s", fslbm_batch_last_error(s->batch[q.mode]));
-        s->used -= q.bytes;
+        freed += q.bytes;
         q.slot = -1;
         finish(s, id, q, RState::Done, FSLBM_OK, t);
         finished++;
     }
     s->held.clear();
+    if (freed) {  // the encoder may take the next queued request
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->used -= freed;
+        }
+        s->cv.notify_one();
+    }
     // 5. hold the requests whose budget is now spent (no further steps)
     std::vector<long long> still;
     still.reserve(s->running.size() + uploaded.size());
@@ -470,6 +570,7 @@ int fslbm_server_drop(fslbm_server_t s, long long id)
     if (it == s->reqs.end()) return sfail(s, FSLBM_EINVAL, "no request %lld", id);
     if (it->second.state != RState::Done && it->second.state != RState::Failed)
         return sfail(s, FSLBM_ESTATE, "request %lld has not finished", id);
+    std::lock_guard<std::mutex> lk(s->mu);
     s->reqs.erase(it);
     return FSLBM_OK;
 }
@@ -478,10 +579,14 @@ int fslbm_server_query(fslbm_server_t s, int *queued, int *running, int *finishe
                        size_t *budget)
 {
     if (!s) return FSLBM_EINVAL;
-    int fin = 0;
-    for (const auto &kv : s->reqs)
-        if (kv.second.state == RState::Done || kv.second.state == RState::Failed) fin++;
-    if (queued) *queued = (int)(s->queue.size() + s->ready.size());
+    std::lock_guard<std::mutex> lk(s->mu);
+    int fin = 0, wait = 0;
+    for (const auto &kv : s->reqs) {
+        const RState st = kv.second.state;
+        fin += st == RState::Done || st == RState::Failed;
+        wait += st == RState::Queued || st == RState::Encoding || st == RState::Ready;
+    }
+    if (queued) *queued = wait;
     if (running) *running = (int)(s->running.size() + s->held.size());
     if (finished) *finished = fin;
     if (used) *used = s->used;

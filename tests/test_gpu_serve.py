@@ -77,21 +77,27 @@ def test_results_equal_solo_runs(policy, fill):
 
 
 def test_arrivals_while_running_join_the_batch():
-    """A request submitted while others are resident is admitted at the
-    next pump and its result is unchanged by the company it ran in."""
+    """A request submitted while others are resident is encoded by the
+    server's encoder thread meanwhile and admitted at a pump after that,
+    while the first is still running; its result is unchanged by the
+    company it ran in."""
     (prmA, mA, sA), _, _ = REQS[0]
     (prmB, mB, sB), _, _ = REQS[2]
     with Server(d1=2, hbm_budget=8 << 30, quantum=8) as srv:
-        a = srv.submit(prmA, sA, mA, "f32", 40)
-        srv.pump()
+        a = srv.submit(prmA, sA, mA, "f32", 400)
+        srv.pump()  # nothing to step: waits for the encoder, then uploads a
+        assert srv.query()["running"] == 1
         srv.pump()
         b = srv.submit(prmB, sB, mB, "f32", 30)
-        srv.pump()
+        for _ in range(20):  # a has 49 pumps left
+            srv.pump()
+            if srv.query()["running"] == 2:
+                break
         assert srv.query()["running"] == 2
         done = {c.id: c for c in drain(srv)}
         assert done[a].ok and done[b].ok
         assert done[b].t_admit > done[a].t_admit
-        for rid, (prm, mask, st), steps in ((a, (prmA, mA, sA), 40), (b, (prmB, mB, sB), 30)):
+        for rid, (prm, mask, st), steps in ((a, (prmA, mA, sA), 400), (b, (prmB, mB, sB), 30)):
             ref = alone(prm, mask, st, steps, "f32")
             assert all(np.array_equal(g, e) for g, e in zip(srv.fetch(rid), ref))
 
@@ -100,8 +106,11 @@ def test_budget_admits_in_arrival_order():
     (prm, mask, st), _, _ = REQS[2]
     need = Batch.domain_bytes("f32", np.shape(st[0]), True)
     with Server(d1=2, hbm_budget=int(2.5 * need), quantum=10) as srv:
-        ids = [srv.submit(prm, st, mask, "f32", 20) for _ in range(3)]
-        srv.pump()
+        ids = [srv.submit(prm, st, mask, "f32", 200) for _ in range(3)]
+        for _ in range(10):  # the encoder readies the first two in turn
+            srv.pump()
+            if srv.query()["running"] == 2:
+                break
         q = srv.query()
         assert q["running"] == 2 and q["queued"] == 1 and q["used"] == 2 * need
         done = {c.id: c for c in drain(srv)}
@@ -129,7 +138,7 @@ def test_invalid_requests():
             srv.set_policy("shortest", -1)
         from flowcontrast_amd._lib import lib
         assert lib().fslbm_server_set_policy(srv._h, 7, 0) < 0
-        # too short for the tiled kernels: fails at admission, alone
+        # too short for the tiled kernels: fails alone, at submission
         short = [a[:8] for a in st]
         bad = srv.submit(prm, short, mask[:8], "f32", 10)
         good = srv.submit(prm, st, mask, "f32", 10)
