@@ -201,12 +201,14 @@ struct Store<FSLBM_BF16S> {
     }
     __host__ __device__ static S st_u(R v, const Quant &) { return f32_to_bf16_bits(v); }
     static constexpr bool SR = true;
+    // Stochastic rounding to bfloat16 as integer arithmetic: the top 16
+    // bits of r added to the float's bits below the bfloat16 mantissa carry
+    // into it with probability (dropped bits) / 2^16, then the top half is
+    // kept (a carry out of the mantissa moves to the next binade, as it
+    // should; finite values stay finite short of the largest binade).
     __device__ static S sr(float x, uint32_t r)
     {
-        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-        b2 t = {0, 0};
-        t = __builtin_amdgcn_cvt_sr_bf16_f32(t, x, r, false);
-        return __builtin_bit_cast(uint16_t, t.x);
+        return (S)((__builtin_bit_cast(uint32_t, x) + (r >> 16)) >> 16);
     }
     __device__ static S st_p_sr(R v, int z, const Quant &q, uint32_t r)
     {
@@ -219,11 +221,9 @@ struct Store<FSLBM_BF16S> {
     }
     __device__ static uint32_t sr2(float lo, uint32_t rlo, float hi, uint32_t rhi)
     {
-        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-        b2 t = __builtin_bit_cast(b2, lo);  // both halves are written below
-        t = __builtin_amdgcn_cvt_sr_bf16_f32(t, lo, rlo, false);
-        t = __builtin_amdgcn_cvt_sr_bf16_f32(t, hi, rhi, true);
-        return __builtin_bit_cast(uint32_t, t);
+        const uint32_t a = __builtin_bit_cast(uint32_t, lo) + (rlo >> 16);
+        const uint32_t b = __builtin_bit_cast(uint32_t, hi) + (rhi >> 16);
+        return __builtin_amdgcn_perm(b, a, 0x07060302u);  // (b.hi << 16) | a.hi
     }
 };
 
