@@ -529,20 +529,27 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
         for (int t = 0; t < 2 * trial_steps; t++)
             HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
     }
-    for (int cand : cands) {
-        const TiledAux aux = tiled_aux(c, std::min(cand, c->g.nz));
-        // warm-up launch, then timed launches
-        HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
-        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        for (int t = 0; t < trial_steps; t++)
+    // Two passes over the candidates, each candidate keeping its faster
+    // time.  With one timing each, run-to-run noise of about 1% made two
+    // of seven FP16-storage runs on the 256x256x1024 pipe pick 342-plane
+    // chunks, which then ran 2-3% slower than the 79-plane ones.
+    std::vector<double> tbest(cands.size(), 1e300);
+    for (int pass = 0; pass < 2; pass++)
+        for (size_t k = 0; k < cands.size(); k++) {
+            const TiledAux aux = tiled_aux(c, std::min(cands[k], c->g.nz));
+            // warm-up launch, then timed launches
             HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
-        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev1));
-        float ms = 0;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        const double per = ms / trial_steps;
-        if (per < bms) { bms = per; best = std::min(cand, c->g.nz); }
-    }
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            for (int t = 0; t < trial_steps; t++)
+                HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            tbest[k] = std::min(tbest[k], (double)ms / trial_steps);
+        }
+    for (size_t k = 0; k < cands.size(); k++)
+        if (tbest[k] < bms) { bms = tbest[k]; best = std::min(cands[k], c->g.nz); }
     c->chunk = best;
     if (best_chunk) *best_chunk = best;
     if (best_ms) *best_ms = bms;
