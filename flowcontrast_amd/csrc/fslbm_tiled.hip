@@ -85,6 +85,10 @@ constexpr double F32_BOX_ROWS_COST = 0.79;
 // storage mode: the code queues and the wall-weight arithmetic need more
 // registers than 12-row tiles (15 waves, 128 registers) leave.
 constexpr int M3_ROWS = 8;
+// FP64 on the pipelined kernel: 4-row tiles, 7 waves (two per SIMD) in
+// 238 registers without spills; at 6 or 8 rows (three waves per SIMD, at
+// most 168 registers) it spills 380 bytes per lane.
+constexpr int F64_PIPE_ROWS = 4;
 
 template <typename R>
 struct Msg {
@@ -242,6 +246,11 @@ __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
 // With NT, FP32 state is stored non-temporal (cache-policy bit NT):
 // nothing in the launch reads it back, so its lines do not displace the
 // input planes that neighbouring tiles still read from L2 (halo rows and
@@ -258,6 +267,12 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsign
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff,
                                           NT ? STORE_NT : 0);
+}
+template <bool NT>
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, NT ? STORE_NT : 0);
 }
 template <bool NT>
 __device__ __forceinline__ void bstore(_Float16 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
@@ -597,6 +612,17 @@ __device__ __forceinline__ float lane_shift(float src, float edge)
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
         __builtin_bit_cast(int, edge), __builtin_bit_cast(int, src), CTRL, 0xf, 0xf, false));
 }
+// the same for a double: one DPP move per 32-bit half
+template <int CTRL>
+__device__ __forceinline__ double lane_shift(double src, double edge)
+{
+    typedef int i2 __attribute__((ext_vector_type(2)));
+    const i2 sv = __builtin_bit_cast(i2, src), ev = __builtin_bit_cast(i2, edge);
+    i2 r;
+    r.x = __builtin_amdgcn_update_dpp(ev.x, sv.x, CTRL, 0xf, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(ev.y, sv.y, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, r);
+}
 
 // Per-cell wall codes of a mask that varies along z (k_wall_codes): the
 // wall rule's weights of one cell, omega = sum over its wall neighbours of
@@ -639,7 +665,6 @@ __device__ __forceinline__ void pipe_item(
     using ST = Store<MODE>;
     using R = typename ST::R;
     using S = typename ST::S;
-    static_assert(sizeof(R) == 4, "pipelined kernel takes 32-bit arithmetic");
     static_assert(!(M3 && PERIODIC), "a periodic box has no mask");
     // Corrector reach: d1, and 2*d1 when the order-4 arms carry weight
     // (blend != 0).  The z queue holds the planes c-HF .. a+HF+LL of the
@@ -780,9 +805,11 @@ __device__ __forceinline__ void pipe_item(
     // The physical plane of plane zl (z wraps) and the pressure jump of the
     // wrap.  zl is the same on every lane: the jump is selected as bits,
     // which keeps the selects on the scalar unit.
+    using RB = std::conditional_t<sizeof(R) == 8, uint64_t, uint32_t>;  // R's bits
     auto phys = [&](int zl, R &jump) -> int {
-        const uint32_t pb = __builtin_bit_cast(uint32_t, P.pin);
-        const uint32_t jb = zl < 0 ? pb : zl >= nz ? pb ^ 0x80000000u : 0u;
+        const RB pb = __builtin_bit_cast(RB, P.pin);
+        const RB sign = RB(1) << (8 * sizeof(R) - 1);
+        const RB jb = zl < 0 ? pb : zl >= nz ? pb ^ sign : RB(0);
         jump = __builtin_bit_cast(R, jb);
         return zl < 0 ? zl + nz : zl >= nz ? zl - nz : zl;
     };
@@ -1534,13 +1561,13 @@ hipError_t launch_pipe_nts(const void *in, void *out, const TiledAux &aux,
     return hipGetLastError();
 }
 
-// FP32 steps store non-temporal when the domain's state exceeds the
-// last-level cache (stream_stores); the 16-bit modes never do.
+// FP32 and FP64 steps store non-temporal when the domain's state exceeds
+// the last-level cache (stream_stores); the 16-bit modes never do.
 template <int MODE, bool PERIODIC, int D1, int TYP, bool M3>
 hipError_t launch_pipe(const Geo &g, const void *in, void *out, const TiledAux &aux,
                        const TiledParams<typename Store<MODE>::R> &P, int grid, hipStream_t s)
 {
-    if constexpr (MODE == FSLBM_F32)
+    if constexpr (MODE == FSLBM_F32 || MODE == FSLBM_F64)
         if (stream_stores(2 * state_bytes(g, MODE)))
             return launch_pipe_nts<MODE, PERIODIC, D1, TYP, true, M3>(in, out, aux, P, grid, s);
     return launch_pipe_nts<MODE, PERIODIC, D1, TYP, false, M3>(in, out, aux, P, grid, s);
@@ -1555,10 +1582,14 @@ hipError_t launch_t(const Geo &g, const void *in, void *out,
     using S = typename Store<MODE>::S;
     using R = typename Store<MODE>::R;
     const int grid = 8 * P.per_xcd;
-    // FP64 messages would not fit double-buffered in LDS: FP64 runs the
-    // two-barrier kernel, the 32-bit-arithmetic modes the pipelined one.
+    // FP64 runs the pipelined kernel on 4-row tiles where its buffer
+    // offsets reach (f64_pipe_fits), else the two-barrier kernel, whose
+    // field offsets are 64-bit.
     if constexpr (MODE == FSLBM_F64) {
         if (aux.wcode) return hipErrorInvalidValue;  // z-invariant masks only
+        if (aux.rows == F64_PIPE_ROWS)
+            return launch_pipe<MODE, PERIODIC, D1, F64_PIPE_ROWS, false>(g, in, out, aux, P, grid, s);
+        if (aux.rows != TY) return hipErrorInvalidValue;
         k_step_tiled<MODE, PERIODIC, D1><<<grid, NT, 0, s>>>(
             (const S *)in, (S *)out, aux.mask2d, (const R *)aux.omega, P);
     }
@@ -1639,9 +1670,18 @@ double chunk_cost(int nz, long long ntile, int nch, int cus)
 
 }  // namespace
 
+// The pipelined kernel's buffer offsets reach an FP64 domain: each field
+// below 2^32 bytes (2^31 in a periodic box, whose out-of-box lanes store at
+// an offset past 2^31 that the range check drops).
+static bool f64_pipe_fits(const Geo &g)
+{
+    const unsigned long long bytes = 8ull * (unsigned long long)g.N;
+    return bytes < (g.periodic ? (1ull << 31) : (1ull << 32));
+}
+
 int plan_rows(int precision, bool periodic, bool order4, bool zvar)
 {
-    if (precision == FSLBM_F64) return TY;
+    if (precision == FSLBM_F64) return F64_PIPE_ROWS;
     if (zvar && !periodic) return M3_ROWS;
     if ((precision == FSLBM_F16S || precision == FSLBM_BF16S) && !periodic && !order4)
         return F16S_WALLED_ROWS;
@@ -1650,6 +1690,8 @@ int plan_rows(int precision, bool periodic, bool order4, bool zvar)
 
 int solo_rows(int precision, const Geo &g, bool order4, bool zvar)
 {
+    // FP64 past the pipelined kernel's offsets: the two-barrier kernel
+    if (precision == FSLBM_F64 && !f64_pipe_fits(g)) return TY;
     const int rows = plan_rows(precision, g.periodic != 0, order4, zvar);
     if (precision != FSLBM_F32 || !g.periodic) return rows;
     // every tile of a periodic box is stepped: compare the modelled step
