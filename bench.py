@@ -607,19 +607,19 @@ def bench_rough(steps: int, warmup: int, precisions=("f32", "f16s", "bf16s"),
 
 
 def bench_storage_modes(mask, st, prm, steps: int, warmup: int):
-    """The 16-bit storage modes (FP32 arithmetic) on the same pipe and
-    initial state as the FP32 line: step rate, roofline fraction over
-    their 17 bytes per cell, and the ratio to an FP32 run timed the same
-    way.  Each mode is timed over at least 200 steps after at least 50
+    """The 16-bit storage modes (FP32 arithmetic) and FP64 (the parity
+    path) on the same pipe and initial state as the FP32 line: step rate,
+    roofline fraction over their bytes per cell (17, 65), and the ratio to
+    an FP32 run timed the same way.  Each mode is timed over at least 200 steps after at least 50
     warm-up steps (the ratio of rates over a few steps is mostly noise)."""
     from flowcontrast_amd import KERNEL_AUTO, Solver
     n = mask.size
     steps, warmup = max(steps, 200), max(warmup, 50)
     out = {"workload": "the FP32 line's pipe and initial state", "steps": steps,
            "warmup": warmup}
-    for prec in ("f32", "f16s", "bf16s"):
+    for prec in ("f32", "f16s", "bf16s", "f64"):
         with Solver(mask.shape, prm, mask=mask, precision=prec, kernel=KERNEL_AUTO) as s:
-            s.set_state(*[a.astype(np.float32) for a in st])
+            s.set_state(*(st if prec == "f64" else [a.astype(np.float32) for a in st]))
             s.autotune(50)  # on the uploaded state, right before the warm-up
             s.step(warmup)
             s.sync()

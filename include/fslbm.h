@@ -115,13 +115,14 @@ long long fslbm_step_cells(fslbm_t ctx);
  * blend: the corrector's order-4 weight the domain is created with.
  * mask2d: one z plane of nx*ny codes (0/125), or NULL for a periodic box.
  * Outputs, any of which may be NULL: the rows of the tiles (64 columns
- * wide; 8 for FP64 and BF16S; 12 for FP32, or 8 in a periodic box where the
- * step-time model rates 8-row tiles faster; for FP16S 12 in a walled box
- * created with blend 0, else 8), the number of tiles holding fluid
+ * wide; 12 for FP32, or 8 in a periodic box where the step-time model
+ * rates 8-row tiles faster; for FP16S and BF16S 12 in a walled box created
+ * with blend 0, else 8; 8 for a mask that varies along z; 4 for FP64, or 8
+ * for an FP64 domain past the pipelined kernel's 32-bit buffer offsets,
+ * which runs the two-barrier kernel), the number of tiles holding fluid
  * (work items run on these only), the cells one step moves (as
  * fslbm_step_cells) and the default chunk (z planes per work item).  A
- * context keeps the tiles it was created with when its blend changes.
- * FP64 runs the two-barrier kernel on 8-row tiles. */
+ * context keeps the tiles it was created with when its blend changes. */
 int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
                     const uint8_t *mask2d, int *tile_rows, int *ntiles, long long *cells,
                     int *chunk);
