@@ -128,11 +128,11 @@ int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
                     int *chunk);
 
 /* Autotuner of the tiled kernel's schedule: after 2*trial_steps untimed
- * launches, times trial_steps launches for each candidate chunk (the
+ * launches, times trial_steps launches for each candidate chunk twice (the
  * step-time model's best picks and a few fixed lengths; HIP events on the
- * context's stream), keeps the
- * fastest and reports it (*best_chunk, *best_ms per step; either may be
- * NULL).  The solver state is not advanced: trial launches read the
+ * context's stream), keeps the fastest (the model's first pick unless
+ * another is at least 2% faster) and reports it (*best_chunk, *best_ms per
+ * step; either may be NULL).  The solver state is not advanced: trial launches read the
  * current state and write the spare buffer.  No-op for the plain kernel. */
 int fslbm_autotune(fslbm_t ctx, int trial_steps, int *best_chunk,
                    double *best_ms);
