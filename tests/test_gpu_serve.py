@@ -84,12 +84,12 @@ def test_arrivals_while_running_join_the_batch():
     (prmA, mA, sA), _, _ = REQS[0]
     (prmB, mB, sB), _, _ = REQS[2]
     with Server(d1=2, hbm_budget=8 << 30, quantum=8) as srv:
-        a = srv.submit(prmA, sA, mA, "f32", 400)
+        a = srv.submit(prmA, sA, mA, "f32", 4000)
         srv.pump()  # nothing to step: waits for the encoder, then uploads a
         assert srv.query()["running"] == 1
         srv.pump()
         b = srv.submit(prmB, sB, mB, "f32", 30)
-        for _ in range(20):  # a has 49 pumps left
+        for _ in range(400):  # a has 498 pumps left
             srv.pump()
             if srv.query()["running"] == 2:
                 break
@@ -97,7 +97,7 @@ def test_arrivals_while_running_join_the_batch():
         done = {c.id: c for c in drain(srv)}
         assert done[a].ok and done[b].ok
         assert done[b].t_admit > done[a].t_admit
-        for rid, (prm, mask, st), steps in ((a, (prmA, mA, sA), 400), (b, (prmB, mB, sB), 30)):
+        for rid, (prm, mask, st), steps in ((a, (prmA, mA, sA), 4000), (b, (prmB, mB, sB), 30)):
             ref = alone(prm, mask, st, steps, "f32")
             assert all(np.array_equal(g, e) for g, e in zip(srv.fetch(rid), ref))
 
@@ -106,8 +106,8 @@ def test_budget_admits_in_arrival_order():
     (prm, mask, st), _, _ = REQS[2]
     need = Batch.domain_bytes("f32", np.shape(st[0]), True)
     with Server(d1=2, hbm_budget=int(2.5 * need), quantum=10) as srv:
-        ids = [srv.submit(prm, st, mask, "f32", 200) for _ in range(3)]
-        for _ in range(10):  # the encoder readies the first two in turn
+        ids = [srv.submit(prm, st, mask, "f32", 2000) for _ in range(3)]
+        for _ in range(150):  # the encoder readies the first two in turn
             srv.pump()
             if srv.query()["running"] == 2:
                 break
