@@ -329,12 +329,15 @@ int fslbm_batch_sync(fslbm_batch_t b);
 /* Each pump advances every stepping request by up to `quantum` steps   */
 /* in one batched launch sequence per precision mode (the modes overlap */
 /* on separate streams).  Meanwhile it copies out the results of the    */
-/* requests whose budget ran out in the previous pump, and encodes and  */
-/* uploads queued requests in arrival order while the HBM budget holds  */
-/* them; the uploaded requests join the stepping ones at the next pump  */
-/* (continuous batching).  A request holds device memory from upload to */
-/* hand-back.  Its result is bitwise the same as a solo run of the      */
-/* tiled single-domain path in its precision mode.                     */
+/* requests whose budget ran out in the previous pump, and uploads the  */
+/* requests that the server's encoder thread has encoded (in arrival    */
+/* order, while the HBM budget holds them); the uploaded requests join  */
+/* the stepping ones at the next pump (continuous batching).  A request */
+/* counts against the budget from the start of its encoding and holds   */
+/* device memory from upload to hand-back.  Its result is bitwise the   */
+/* same as a solo run of the tiled single-domain path in its precision  */
+/* mode.  The calls on one server come from one host thread at a time   */
+/* (the encoder thread is internal).                                    */
 /* ------------------------------------------------------------------ */
 typedef struct fslbm_server *fslbm_server_t;
 
