@@ -215,6 +215,9 @@ TiledAux tiled_aux(const fslbm_ctx *c, int chunk)
 {
     TiledAux aux{c->mask2d, c->omega, c->tiles, c->ntiles, chunk, c->rows, c->stream};
     aux.wcode = c->wcode;
+    // the buffers are zeroed at creation, every upload (state, snapshot)
+    // writes both with walls stored as 0, and every step stores 0 at walls
+    aux.walls_zero = true;
     return aux;
 }
 

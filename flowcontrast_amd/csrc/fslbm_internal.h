@@ -94,6 +94,10 @@ struct TiledAux {
     // per-cell wall codes of a mask that varies along z (launch_wall_codes;
     // nullptr: z-invariant mask or periodic box, mask2d and omega instead)
     const uint16_t *wcode = nullptr;
+    // both state buffers hold zero at every wall cell (a context keeps that
+    // from the upload on): the kernel may skip the loads and stores of
+    // rows of wall cells
+    bool walls_zero = false;
 };
 // Wall codes (2 bytes per cell) of a walled box's mask, for the tiled
 // kernel's variant for masks that vary along z.

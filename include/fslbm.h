@@ -103,11 +103,13 @@ int fslbm_get_kernel(fslbm_t ctx);
 int fslbm_set_tile_chunk(fslbm_t ctx, int chunk);
 int fslbm_get_tile_chunk(fslbm_t ctx);
 
-/* Lattice cells one step reads and writes (no reference counterpart; for
+/* Lattice cells one step computes (no reference counterpart; for
  * bandwidth accounting).  The tiled kernel skips tiles that hold only wall
  * cells (their state stays zero), so this is the cell count of the tiles
- * holding fluid; the plain kernel covers all nx*ny*nz cells.  Returns a
- * negative error code on failure. */
+ * holding fluid; the plain kernel covers all nx*ny*nz cells.  Inside those
+ * tiles the tiled kernel also neither reads nor writes the 16-cell row
+ * pieces of a walled box whose cells are all walls.  Returns a negative
+ * error code on failure. */
 long long fslbm_step_cells(fslbm_t ctx);
 
 /* Host-side plan of the tiled kernel's schedule for a domain (no reference
@@ -120,7 +122,7 @@ long long fslbm_step_cells(fslbm_t ctx);
  * with blend 0, else 8; 8 for a mask that varies along z; 4 for FP64, or 8
  * for an FP64 domain past the pipelined kernel's 32-bit buffer offsets,
  * which runs the two-barrier kernel), the number of tiles holding fluid
- * (work items run on these only), the cells one step moves (as
+ * (work items run on these only), the cells one step computes (as
  * fslbm_step_cells) and the default chunk (z planes per work item).  A
  * context keeps the tiles it was created with when its blend changes. */
 int fslbm_tile_plan(int nx, int ny, int nz, int precision, double blend,
