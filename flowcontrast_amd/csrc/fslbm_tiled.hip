@@ -738,6 +738,17 @@ __device__ __forceinline__ void pipe_item(
             return wskip && q == 0xffffu;
         }
     };
+    // the same for a single cell's loads (the halo column wave's scattered
+    // reads): since the neighbouring tiles skip their wall quarter-rows,
+    // such a read would fetch a line for its one zero
+    auto wall_cell = [&](unsigned c) __attribute__((always_inline)) -> bool {
+        if constexpr (PERIODIC || M3) {
+            (void)c;
+            return false;
+        } else {
+            return wskip && mask2d[c] != 0;
+        }
+    };
     // fields 0, 1 (k = 0) or 2, 3 (k = 1) of the cell at column offset cb
     // and plane offset so
     auto ldpair = [&](int k, unsigned cb, unsigned so) __attribute__((always_inline)) -> S2 {
@@ -1336,7 +1347,7 @@ __device__ __forceinline__ void pipe_item(
         const unsigned col = colx(x) + nx * coly(y);
         const bool fluid = PERIODIC ? true : M3 ? inside : (inside && mask2d[col] == 0);
         const bool writer = lane < 2 * RYP;
-        const unsigned cb_ = col * CB;
+        const unsigned cb_ = wall_cell(col) ? OOB : col * CB;
         S2 ha[PD], hb[PD];
         uint32_t hc[PD];  // wall codes (M3)
         auto hload = [&](int m, int s) __attribute__((always_inline)) {
@@ -1364,7 +1375,7 @@ __device__ __forceinline__ void pipe_item(
             afy[j] = HF + br;
             afx[j] = bc < HF ? bc : TX + bc;
             const unsigned ac = colx(x0 - HF + afx[j]) + nx * coly(y0 - HF + afy[j]);
-            acol[j] = ac * CB;
+            acol[j] = wall_cell(ac) ? OOB : ac * CB;
             aref[j] = uref_of(ac);
         }
         AV ar[AD][XPL][NA];
