@@ -561,6 +561,16 @@ int fslbm_autotune(fslbm_t c, int trial_steps, int *best_chunk, double *best_ms)
         best = std::min(cands[0], c->g.nz);
     }
     c->chunk = best;
+    // Untimed launches of the chosen schedule (queued, not waited for): the
+    // last candidates are long chunks that fill the GPU's rounds partly, and
+    // after them the chosen schedule ran its first 20-40 steps 2-3% slower
+    // (the GPU's clocks settling to the higher load) on the 256x256x1024
+    // pipe.  These leave the GPU in the chosen schedule's steady state.
+    {
+        const TiledAux aux = tiled_aux(c, best);
+        for (int t = 0; t < 2 * trial_steps; t++)
+            HIPCHK(c, launch_step_tiled(c->g, c->precision, c->state[c->cur], c->state[c->cur ^ 1], aux, a));
+    }
     if (best_chunk) *best_chunk = best;
     if (best_ms) *best_ms = bms;
     return FSLBM_OK;

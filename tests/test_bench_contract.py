@@ -45,8 +45,16 @@ def test_traffic_lookup_matches_profiles():
         workload, prec = key.split(":")
         got = bench.load_traffic(workload, prec)
         assert got == rec["bytes_per_launch"]
-        # measured bytes are at least the algorithmic bytes of the cells moved
-        assert got >= 0.95 * rec["algorithmic_bytes"]
+        # The algorithmic bytes count the cells of the stepped tiles; in a
+        # walled box with a z-invariant mask the kernel neither reads nor
+        # writes those tiles' quarter-rows of wall cells (17.9% of the cells
+        # on the R 120 pipe), so the measured bytes may fall below them, but
+        # not by more than that share.
+        assert got >= 0.8 * rec["algorithmic_bytes"]
+        # the writes: the new state of the same cells, less those quarter-rows
+        state = {"f64": 32, "f32": 16, "f16s": 8, "bf16s": 8}[prec]
+        cells = rec["algorithmic_bytes"] / bench.algorithmic_bytes_per_cell(prec, True)
+        assert 0.8 * state * cells <= rec["write_bytes"] <= 1.01 * state * cells
     assert bench.load_traffic("no_such_workload", "f32") is None
 
 
