@@ -37,7 +37,9 @@ struct fslbm_ctx {
     uint8_t *mask = nullptr;   // device mask (nullptr: periodic box)
     std::vector<uint8_t> hmask;
     bool z_invariant = false;  // every z plane of the mask is the same
-    uint8_t *mask2d = nullptr; // that plane (device), for the tiled kernel
+    // that plane (device), for the tiled kernel; for a mask that varies
+    // along z, the columns that are walls in every plane
+    uint8_t *mask2d = nullptr;
     void *omega = nullptr;     // per-column wall weights (device)
     uint16_t *wcode = nullptr; // per-cell wall codes of a z-varying mask (device)
     float *uref = nullptr;     // streamwise reference profile (16-bit modes, device)
@@ -424,7 +426,13 @@ int fslbm_create(const fslbm_params *prm, const uint8_t *mask, int nx, int ny,
                 for (size_t i = 0; i < nxy; i++)
                     if (mask[z * nxy + i] == 0) proj[i] = 0;
             const std::vector<int> tl = active_tiles(c->g, c->rows, proj.data());
-            if (!tl.empty()) {
+            // for a mask that varies along z the tiled kernel takes the
+            // projection as its plane: the columns that are walls in every
+            // plane, whose loads and stores it skips
+            if (!c->z_invariant)
+                ok = alloc((void **)&c->mask2d, nxy) &&
+                     hipMemcpy(c->mask2d, proj.data(), nxy, hipMemcpyHostToDevice) == hipSuccess;
+            if (ok && !tl.empty()) {
                 ok = alloc((void **)&c->tiles, tl.size() * sizeof(int)) &&
                      hipMemcpy(c->tiles, tl.data(), tl.size() * sizeof(int),
                                hipMemcpyHostToDevice) == hipSuccess;

@@ -712,9 +712,9 @@ __device__ __forceinline__ void pipe_item(
     const __amdgpu_buffer_rsrc_t rU = PR ? rP : mkbuf(in + N, fbytes),
                                  rW = PR ? rV : mkbuf(in + 3ll * N, fbytes);
     auto poff = [&](int zp) { return (unsigned)zp * (unsigned)nxy * CB; };
-    // Quarter-rows of wall columns.  In a walled box with a z-invariant
-    // mask, a row wave's quarter (16 lanes: 64 B of a 32-bit field, 128 B
-    // of an FP64 one) whose columns are all walls reads zeros and stores
+    // Quarter-rows of wall columns.  In a walled box, a row wave's quarter
+    // (16 lanes: 64 B of a 32-bit field, 128 B of an FP64 one) whose
+    // columns are walls in every plane reads zeros and stores
     // zeros over the zeros both buffers hold at walls (P.wskip: contexts
     // and batches keep that from the upload on).  Its loads and stores then
     // take an offset past the field, where the buffer range check returns
@@ -724,25 +724,32 @@ __device__ __forceinline__ void pipe_item(
     // 256x256x1024 pipe (R 120) 17.9% of the cells of the stepped tiles lie
     // in such quarters (12.8% in all-wall halves): the FP32 step runs 4.5%
     // faster, the 16-bit ones 2-3% (half-rows: 4% in FP32).
+    // With a mask that varies along z (M3), mask2d, where given, marks the
+    // columns that are walls in every plane; the 16-bit M3 variants, bound
+    // by vector issue, measured 0.6% slower with the skip on the rough
+    // 256x256x1024 pipe (FP32 5.4% faster) and keep every access.
     constexpr unsigned OOB = 0x80000000u;
-    const bool wskip = !PERIODIC && !M3 && P.wskip != 0 && fbytes <= OOB;
+    constexpr bool SKIP = !PERIODIC && !(M3 && PR);
+    const bool wskip = SKIP && P.wskip != 0 && fbytes <= OOB && mask2d != nullptr;
     // whether the quarter-row of this lane's wave holding column c (the
-    // one the lane loads) is all walls; called by full waves only
+    // one the lane loads) is all walls; called by full waves only (wskip
+    // is the same on every lane)
     auto wall_quarter = [&](unsigned c) __attribute__((always_inline)) -> bool {
-        if constexpr (PERIODIC || M3) {
+        if constexpr (!SKIP) {
             (void)c;
             return false;
         } else {
+            if (!wskip) return false;
             const uint64_t b = __builtin_amdgcn_ballot_w64(mask2d[c] != 0);
             const uint32_t q = (uint32_t)(b >> (16 * ((threadIdx.x & 63) >> 4))) & 0xffffu;
-            return wskip && q == 0xffffu;
+            return q == 0xffffu;
         }
     };
     // the same for a single cell's loads (the halo column wave's scattered
     // reads): since the neighbouring tiles skip their wall quarter-rows,
     // such a read would fetch a line for its one zero
     auto wall_cell = [&](unsigned c) __attribute__((always_inline)) -> bool {
-        if constexpr (PERIODIC || M3) {
+        if constexpr (!SKIP) {
             (void)c;
             return false;
         } else {
