@@ -157,10 +157,11 @@ def field_errors(got, ref):
                             if abs(uz_r) > 1e-3 * rms else None)}
 
 
-def bench_batch(count: int, steps: int, sequential: bool):
+def bench_batch(count: int, steps: int, sequential: bool, graph: bool = True):
     """Independent uneven pipes in one batch (one launch per step over a
     table of (pipe, tile, z range) items, autotuned longest-item cap,
-    hipGraph replay) against the same pipes solved one after another by
+    hipGraph replay, or direct launches with graph False) against the same
+    pipes solved one after another by
     the single-domain solver.  Timed: the steps only, states resident on
     the device on both sides."""
     from flowcontrast_amd import KERNEL_AUTO, KERNEL_PLAIN, Batch, Solver
@@ -176,7 +177,7 @@ def bench_batch(count: int, steps: int, sequential: bool):
             prm, mask, st = c.build()
             slots.append(b.add(prm, st, mask))
         upload = time.perf_counter() - t
-        b.use_graph(True)
+        b.use_graph(graph)
         chunk, _ = b.autotune(10)
         b.sync()
         t0 = time.perf_counter()
@@ -189,7 +190,7 @@ def bench_batch(count: int, steps: int, sequential: bool):
            "pipes": count, "cells": cells, "steps_per_solve": steps,
            "solves_per_s": round(count / wall, 4),
            "mlups": round(cells * steps / wall / 1e6, 1),
-           "seconds": round(wall, 3), "chunk": chunk, "graph": True,
+           "seconds": round(wall, 3), "chunk": chunk, "graph": graph,
            "upload_s": round(upload, 2), "finite": bool(finite)}
     if sequential:
         # the same pipes one after another through the single-domain
@@ -718,6 +719,8 @@ def main() -> int:
                     help="--workload rough: plain-kernel steps timed (0: none)")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10000)
+    ap.add_argument("--batch-direct", action="store_true",
+                    help="--workload batch: direct launches instead of hipGraph replay")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=1.0)
     ap.add_argument("--serve-quantum", type=int, default=200)
@@ -763,7 +766,7 @@ def main() -> int:
 
     if args.workload == "batch":
         print(json.dumps(bench_batch(args.batch_count, args.batch_steps,
-                                     not args.no_sequential)), flush=True)
+                                     not args.no_sequential, not args.batch_direct)), flush=True)
         return 0
     if args.workload == "tgv":
         print(json.dumps(bench_tgv(256, args.tgv_steps, 250, 100, ["f64", "f32", "f16s", "bf16s"])),

@@ -42,13 +42,29 @@ struct Domain {
     long steps = 0;
     bool held = false;  // resident but no longer stepped (result pending)
     int held_cur = 0;   // its current buffer
-    bool parked = false;  // outside the cohort being stepped (step_cohorts)
-    int parked_cur = 0;   // its current buffer while parked
+    int ncols = 0;      // z columns (tiles) of its work items, as of the last rebuild
 };
 
 constexpr int NCLASS = 4;  // (periodic, order-4 arms)
+// Steps per captured graph.  Between two graph launches on a stream the GPU
+// idles about 8 us; with two steps per graph that cost 11% of the 64-pipe
+// batch's cohort steps (37 us launches), with 16 about 1.4%.  Calls of
+// fewer steps, and the remainder of a call, launch directly.
+constexpr int GRAPH_STEPS = 16;
 
 int class_of(const Domain &d) { return (d.g.periodic ? 2 : 0) + (d.prm.blend != 0.0 ? 1 : 0); }
+
+// One cohort of a many-step call (step_impl): its domains (slots) and its
+// own work tables over the batch's descriptors, with their graphs.
+struct Cohort {
+    std::vector<int> doms;
+    int4 *d_work[NCLASS] = {nullptr, nullptr, nullptr, nullptr};
+    size_t cap[NCLASS] = {0, 0, 0, 0};
+    int nwork[NCLASS] = {0, 0, 0, 0};
+    std::vector<int4> host[NCLASS];  // kept until the next rebuild (async upload)
+    bool nt = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // GRAPH_STEPS steps from flip 0 / 1
+};
 
 }  // namespace
 
@@ -69,9 +85,12 @@ struct fslbm_batch {
     int flip = 0;   // steps since the descriptors were written, mod 2
     int chunk = 0;  // schedule (0: default per domain)
     bool use_graph = false;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // two steps from flip 0 / 1
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // GRAPH_STEPS steps from flip 0 / 1
     hipStream_t stream = nullptr;
+    hipStream_t capture = nullptr;  // cohort graphs are captured here, replayed on stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<Cohort> cohorts;
+    bool cohorts_valid = false;  // their tables match the descriptors
     std::string err;
 };
 
@@ -122,6 +141,18 @@ void drop_graphs(fslbm_batch *b)
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto &g : b->gexec)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+// The same for the cohorts' graphs; their tables no longer match.
+void drop_cohort_graphs(fslbm_batch *b)
+{
+    bool any = false;
+    for (const Cohort &co : b->cohorts) any |= co.gexec[0] || co.gexec[1];
+    if (any && b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Cohort &co : b->cohorts)
+        for (auto &g : co.gexec)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    b->cohorts_valid = false;
 }
 
 // One z column of one tile of a domain, still to be cut into work items.
@@ -218,7 +249,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     size_t stepped = 0;  // state bytes the step reads and writes
     for (size_t i = 0; i < nd; i++) {
         Domain &d = b->doms[i];
-        if (!d.active || d.held || d.parked) continue;
+        if (!d.active || d.held) continue;
         stepped += 2 * state_bytes(d.g, b->precision);
         // the current parity of each domain is folded into its descriptor
         d.parity ^= b->flip;
@@ -230,6 +261,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
         seeds[2 * i] = (uint32_t)d.steps;  // the next launch has flip 0
         seeds[2 * i + 1] = (uint32_t)d.steps + 1u;
         auto &cl = cols[class_of(d)];
+        d.ncols = d.htiles.empty() ? split[0] : (int)d.htiles.size();
         if (d.htiles.empty())
             for (int t = 0; t < split[0]; t++) cl.push_back({(int)i, t, 0, d.g.nz});
         else
@@ -245,6 +277,7 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     // rewritten, so that no graph exec outlives memory it refers to.
     BCHK(b, hipStreamSynchronize(b->stream));
     drop_graphs(b);
+    drop_cohort_graphs(b);
     if (host.size() > b->d_doms_cap) {
         if (b->d_doms) (void)hipFree(b->d_doms);
         b->d_doms = nullptr;
@@ -281,31 +314,39 @@ int rebuild(fslbm_batch *b, int chunk_override = -1)
     return FSLBM_OK;
 }
 
-// Launch one step of every class with the given flip (no bookkeeping).
-hipError_t launch_step_all(fslbm_batch *b, int flip)
+// Launch one step of every class of the given work tables with the given
+// flip (no bookkeeping).
+hipError_t launch_tables(const fslbm_batch *b, int4 *const *work, const int *nwork, bool nt,
+                         int flip, hipStream_t s)
 {
     for (int c = 0; c < NCLASS; c++) {
-        if (b->nwork[c] == 0) continue;
+        if (nwork[c] == 0) continue;
         BatchLaunch L;
         L.doms = b->d_doms;
         L.seeds = b->d_seeds;
-        L.work = b->d_work[c];
-        L.nwork = b->nwork[c];
+        L.work = work[c];
+        L.nwork = nwork[c];
         L.flip = flip;
-        L.grid = b->nwork[c];  // one workgroup per item
+        L.grid = nwork[c];  // one workgroup per item
         L.periodic = (c & 2) != 0;
         L.fd2 = (c & 1) != 0;
-        L.nt = b->nt;
+        L.nt = nt;
         L.d1 = b->d1;
         L.precision = b->precision;
-        L.stream = b->stream;
+        L.stream = s;
         const hipError_t e = launch_step_batch(L);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-// Capture the two graphs (two steps each, from flip 0 and from flip 1).
+hipError_t launch_step_all(fslbm_batch *b, int flip)
+{
+    return launch_tables(b, b->d_work, b->nwork, b->nt, flip, b->stream);
+}
+
+// Capture the two graphs (GRAPH_STEPS steps each, from flip 0 and from
+// flip 1).
 // Either both are made or neither: a failure destroys what was made, so
 // that no launch ever finds one graph of the pair missing.
 int capture_graphs(fslbm_batch *b)
@@ -318,8 +359,10 @@ int capture_graphs(fslbm_batch *b)
             drop_graphs(b);
             return bfail(b, FSLBM_EHIP, "graph capture failed: %s", hipGetErrorString(e0));
         }
-        hipError_t e1 = launch_step_all(b, start);
-        hipError_t e2 = launch_step_all(b, start ^ 1);
+        hipError_t e1 = hipSuccess;
+        for (int k = 0; k < GRAPH_STEPS && e1 == hipSuccess; k++)
+            e1 = launch_step_all(b, start ^ (k & 1));
+        hipError_t e2 = hipSuccess;
         hipError_t e3 = hipStreamEndCapture(b->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
@@ -347,12 +390,12 @@ int step_all(fslbm_batch *b, int nsteps)
     bool any = false;
     for (int c = 0; c < NCLASS; c++) any |= b->nwork[c] > 0;
     int s = 0;
-    if (any && b->use_graph && nsteps >= 2) {
+    if (any && b->use_graph && nsteps >= GRAPH_STEPS) {
         if (!b->gexec[0] || !b->gexec[1]) {
             const int rc = capture_graphs(b);
             if (rc) return rc;
         }
-        for (; s + 2 <= nsteps; s += 2)
+        for (; s + GRAPH_STEPS <= nsteps; s += GRAPH_STEPS)
             BCHK(b, hipGraphLaunch(b->gexec[b->flip], b->stream));
     }
     for (; s < nsteps; s++) {
@@ -360,7 +403,79 @@ int step_all(fslbm_batch *b, int nsteps)
         b->flip ^= 1;
     }
     for (auto &d : b->doms)
-        if (d.active && !d.held && !d.parked) d.steps += nsteps;
+        if (d.active && !d.held) d.steps += nsteps;
+    return FSLBM_OK;
+}
+
+
+// The work tables of one cohort over the batch's descriptors (written by
+// the last rebuild, which covers every stepping domain), uploaded on the
+// batch's stream.  The caller has waited for the launches that read the
+// cohort's previous tables.
+int build_cohort(fslbm_batch *b, Cohort &co)
+{
+    std::vector<Column> cols[NCLASS];
+    size_t stepped = 0;
+    for (int i : co.doms) {
+        const Domain &d = b->doms[i];
+        stepped += 2 * state_bytes(d.g, b->precision);
+        auto &cl = cols[class_of(d)];
+        if (d.htiles.empty())
+            for (int t = 0; t < d.ncols; t++) cl.push_back({i, t, 0, d.g.nz});
+        else
+            for (int t : d.htiles) cl.push_back({i, t, 0, d.g.nz});
+    }
+    co.nt = stream_stores(stepped);
+    const int cus = device_cus();
+    for (int c = 0; c < NCLASS; c++) {
+        co.host[c] = split_columns(cols[c], b->chunk > 0 ? b->chunk : 128, cus, b->doms.size());
+        const size_t need = std::max<size_t>(co.host[c].size(), 1) * sizeof(int4);
+        if (need > co.cap[c]) {
+            if (co.d_work[c]) (void)hipFree(co.d_work[c]);
+            co.d_work[c] = nullptr;
+            co.cap[c] = 0;
+            BCHK(b, hipMalloc((void **)&co.d_work[c], need));
+            co.cap[c] = need;
+        }
+        if (!co.host[c].empty())
+            BCHK(b, hipMemcpyAsync(co.d_work[c], co.host[c].data(),
+                                   co.host[c].size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   b->stream));
+        co.nwork[c] = (int)co.host[c].size();
+    }
+    return FSLBM_OK;
+}
+
+// Capture a cohort's two graphs (GRAPH_STEPS steps each, from flip 0 and
+// from flip 1) on the capture stream; they are replayed on the batch's
+// stream.
+int capture_cohort(fslbm_batch *b, Cohort &co)
+{
+    // either both graphs are made or neither
+    auto fail_ = [&](const char *what, hipError_t e) {
+        for (auto &g : co.gexec)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        return bfail(b, FSLBM_EHIP, "%s failed: %s", what, hipGetErrorString(e));
+    };
+    for (int start = 0; start < 2; start++) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return fail_("graph capture", e);
+        hipError_t el = hipSuccess;
+        for (int k = 0; k < GRAPH_STEPS && el == hipSuccess; k++)
+            el = launch_tables(b, co.d_work, co.nwork, co.nt, start ^ (k & 1), b->capture);
+        e = hipStreamEndCapture(b->capture, &g);
+        if (el != hipSuccess || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail_("graph capture", el != hipSuccess ? el : e);
+        }
+        e = hipGraphInstantiate(&co.gexec[start], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            co.gexec[start] = nullptr;
+            return fail_("graph instantiate", e);
+        }
+    }
     return FSLBM_OK;
 }
 
@@ -372,8 +487,12 @@ int step_all(fslbm_batch *b, int nsteps)
 // slot order) whose state together fits COHORT_BYTES, and each cohort runs
 // all the steps before the next: every launch still covers many domains
 // (which small domains alone cannot fill the GPU with) and keeps them in
-// cache.  Only the schedule changes: each domain's arithmetic and step
-// counter are those of one batch step after another.
+// cache.  One descriptor table serves every cohort; each cohort has its own
+// work tables (and graphs), written when the cohorts change and otherwise
+// reused, so the cohorts' launches are queued back to back without waiting
+// for the stream (the host builds the next cohort's tables while the GPU
+// runs the previous one).  Only the schedule changes: each domain's
+// arithmetic and step counter are those of one batch step after another.
 constexpr size_t COHORT_BYTES = LLC_BYTES / 4 * 3;
 constexpr int COHORT_MIN_STEPS = 32;
 
@@ -397,28 +516,44 @@ int step_impl(fslbm_batch *b, int nsteps)
         if (!stream_stores(total)) cohorts.clear();  // fits the cache as a whole
     }
     if (cohorts.size() < 2) return step_all(b, nsteps);
-    for (const auto &co : cohorts) {
-        // park every stepping domain outside this cohort at its current
-        // buffer (parity ^ flip, as for a hold), step, then unpark
-        std::vector<char> in(b->doms.size(), 0);
-        for (int i : co) in[i] = 1;
-        for (size_t i = 0; i < b->doms.size(); i++) {
-            Domain &d = b->doms[i];
-            if (!d.active || d.held || in[i]) continue;
-            d.parked_cur = d.parity ^ b->flip;
-            d.parked = true;
-        }
-        b->dirty = true;
-        const int rc = step_all(b, nsteps);
-        for (size_t i = 0; i < b->doms.size(); i++) {
-            Domain &d = b->doms[i];
-            if (!d.parked) continue;
-            d.parity = d.parked_cur ^ b->flip;
-            d.parked = false;
-        }
-        b->dirty = true;
+    if (b->dirty) {  // descriptors of every stepping domain (waits for the stream)
+        const int rc = rebuild(b);
         if (rc) return rc;
     }
+    bool same = b->cohorts_valid && b->cohorts.size() == cohorts.size();
+    for (size_t k = 0; same && k < cohorts.size(); k++) same = b->cohorts[k].doms == cohorts[k];
+    if (!same) {
+        // the launches of the last call may still read the cohort tables
+        BCHK(b, hipStreamSynchronize(b->stream));
+        drop_cohort_graphs(b);
+        for (size_t k = cohorts.size(); k < b->cohorts.size(); k++)
+            for (auto *w : b->cohorts[k].d_work) if (w) (void)hipFree(w);
+        b->cohorts.resize(cohorts.size());
+        for (size_t k = 0; k < cohorts.size(); k++) b->cohorts[k].doms = cohorts[k];
+    }
+    const int f0 = b->flip;
+    for (size_t k = 0; k < cohorts.size(); k++) {
+        Cohort &co = b->cohorts[k];
+        if (!same) {
+            const int rc = build_cohort(b, co);
+            if (rc) return rc;
+        }
+        int s = 0;
+        if (b->use_graph && nsteps >= GRAPH_STEPS) {
+            if (!co.gexec[0] || !co.gexec[1]) {
+                const int rc = capture_cohort(b, co);
+                if (rc) return rc;
+            }
+            for (; s + GRAPH_STEPS <= nsteps; s += GRAPH_STEPS)
+                BCHK(b, hipGraphLaunch(co.gexec[f0], b->stream));
+        }
+        for (; s < nsteps; s++)
+            BCHK(b, launch_tables(b, co.d_work, co.nwork, co.nt, f0 ^ (s & 1), b->stream));
+    }
+    b->cohorts_valid = true;
+    b->flip = f0 ^ (nsteps & 1);
+    for (auto &d : b->doms)
+        if (d.active && !d.held) d.steps += nsteps;
     return FSLBM_OK;
 }
 
@@ -651,6 +786,7 @@ int fslbm_batch_create(int precision, int d1, size_t hbm_budget, fslbm_batch_t *
     b->d1 = d1;
     b->budget = hbm_budget;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         g_batch_err = "stream/event creation failed";
         fslbm_batch_destroy(b);
@@ -665,6 +801,9 @@ void fslbm_batch_destroy(fslbm_batch_t b)
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
+    drop_cohort_graphs(b);
+    for (auto &co : b->cohorts)
+        for (auto *w : co.d_work) if (w) (void)hipFree(w);
     for (auto &d : b->doms) {
         for (void *p : d.buf) if (p) (void)hipFree(p);
         if (d.mask2d) (void)hipFree(d.mask2d);
@@ -678,6 +817,7 @@ void fslbm_batch_destroy(fslbm_batch_t b)
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
     delete b;
 }
 

@@ -311,13 +311,16 @@ int fslbm_batch_get_state(fslbm_batch_t b, int slot, double *p, double *ux,
 int fslbm_batch_query(fslbm_batch_t b, size_t *used, size_t *budget,
                       int *ndomains, long long *cells);
 
-/* Replay the step sequence from a captured hipGraph (two steps per
- * graph launch). */
+/* Replay the step sequence from captured hipGraphs (16 steps per graph
+ * launch; the rest of a call, and calls of fewer steps, launch directly).
+ * A call of 32 steps or more over state larger than the last-level cache
+ * steps the domains in cohorts that fit it, each with its own tables and
+ * graphs, built once per call. */
 int fslbm_batch_use_graph(fslbm_batch_t b, int enable);
 
 /* Schedule: the longest work item in z planes (0: default, 128).  Tile
- * columns are cut into items that shorten as the table fills (the last
- * ones are at most 8 planes) and run longest first.  autotune picks the
+ * columns are cut into near-equal items of one length, the step-time
+ * model's pick up to this cap, and run longest first.  autotune picks the
  * fastest cap by timing trial launches that do not advance any domain. */
 int fslbm_batch_set_chunk(fslbm_batch_t b, int chunk);
 int fslbm_batch_autotune(fslbm_batch_t b, int trial_steps, int *best_chunk,

@@ -43,7 +43,8 @@ CASES = [pipe_case(30, 64, 12.5, 300, 0.01, 1),
 @pytest.mark.parametrize("precision", ["f32", "f16s"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_equals_alone(precision, graph):
-    steps = 13
+    # with graphs: two replays of the 16-step graph and 5 direct launches
+    steps = 37 if graph else 13
     ref = [alone(prm, mask, st, steps, precision) for prm, mask, st in CASES]
     with Batch(precision, 2, 8 << 30) as b:
         b.use_graph(graph)

@@ -63,7 +63,11 @@ def check_f32(orc, got, prm, mask, st32, steps, what):
 
 
 def test_uneven_pipes_batch(orc):
-    steps = 50
+    """The 64 pipes (5 GB of state) step in cohorts that fit the last-level
+    cache.  Two calls: the first (33 steps: two 16-step graph replays and
+    one direct launch per cohort) builds the cohorts' tables and graphs, the
+    second (34 steps) reuses them from the other buffer parity."""
+    steps = 67
     cases = W.uneven_pipes(64, 4, steps)
     assert len(cases) == 64
     with Batch("f32", 2, 64 << 30) as b:
@@ -72,7 +76,8 @@ def test_uneven_pipes_batch(orc):
         for c in cases:
             prm, mask, st = c.build()
             slots.append(b.add(prm, [a.astype(np.float32) for a in st], mask))
-        b.step(steps)
+        b.step(33)
+        b.step(34)
         b.sync()
         smallest = sorted(range(64), key=lambda k: cases[k].cells)[:3]
         for k, (c, sl) in enumerate(zip(cases, slots)):
