@@ -4,11 +4,13 @@
 // for HBM bandwidth:
 //
 // * One workgroup owns a 64 x TY (x, y) tile and marches over a chunk of
-//   z planes: TY = 8 in the two-barrier FP64 kernel (k_step_tiled), 12
-//   (FP32, and FP16 storage in a walled box at blend 0) or 8 (other 16-bit
-//   storage) in the pipelined kernel (k_step_pipe, further below).  Every
-//   state value is written once per step and read from HBM about once
-//   (1.04-1.07x: halo and chunk-end re-reads mostly hit L2).
+//   z planes: TY = 8 in the two-barrier FP64 kernel (k_step_tiled); in
+//   the pipelined kernel (k_step_pipe, further below) 12 (FP32, and the
+//   16-bit storage modes in a walled box at blend 0), 8 (other 16-bit
+//   variants, masks that vary along z) or 4 (FP64).  Every state value is
+//   written once per step and read from HBM about once (halo and chunk-end
+//   re-reads mostly hit L2), except wall cells, whose zeros the pipelined
+//   kernel does not move where a 16-cell row piece is all walls.
 // * The 27-point D3Q27 predictor is evaluated as three separable 3-tap
 //   passes.  D3Q27 weights are products w(cx) w(cy) w(cz) with
 //   w(0) = 2/3, w(+-1) = 1/6, and the equilibrium
