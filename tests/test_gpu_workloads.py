@@ -91,12 +91,17 @@ def test_uneven_pipes_batch(orc):
                 check_f32(orc, got, prm, mask, st32, steps, (k, c.shape))
 
 
-def test_serving_stream(orc):
+@pytest.mark.parametrize("policy", ["share", "shortest"])
+def test_serving_stream(orc, policy):
+    """All 32 requests at once: the resident state exceeds the last-level
+    cache, so each pump's batches step in cohorts; under the shortest-work
+    policy requests are also held and resumed between pumps."""
     cases = W.serving_requests(32, 5, 1.0)
     for c in cases:  # step budgets cut 100-fold: 20 to 200 steps
         c.steps = max(2, c.steps // 100)
     built = {}
     with Server(d1=2, hbm_budget=32 << 30, quantum=64) as srv:
+        srv.set_policy(policy, 0)
         owner = {}
         for k, c in enumerate(cases):
             prm, mask, st = c.build()
