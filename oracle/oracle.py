@@ -1,8 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's CPU-baseline leg, as the checker.  The product library never
-loads it.
+bench.py's CPU-baseline leg, as the reference that GPU results are
+compared with.  The product library never loads it.
 
 Arrays are numpy arrays of shape (nz, ny, nx) (x fastest); a state is a
 list [p, ux, uy, uz].
