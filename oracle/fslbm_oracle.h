@@ -2,8 +2,8 @@
  * CPU oracle for the fractional-step lattice Boltzmann (FSLBM) time step.
  *
  * TEST INFRASTRUCTURE ONLY.  This library is what GPU results are compared
- * with: the GPU library never links it, and only tests/, __graft_entry__.smoke() and bench.py's
- * CPU-baseline leg may load it.
+ * with: the GPU library never links it, and only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.
  *
  * It restates the reference update of alparmat/FlowContrast
  * (PipeDemoStandalone/FSLBM.cu) as plain C with OpenMP:
