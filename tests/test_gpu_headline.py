@@ -1,10 +1,12 @@
 """GPU parity at the benchmark sizes, against the CPU oracle directly.
 
 The 256x256x1024 pipe (radius 120, bulk Re 5300, Blasius pin, Poiseuille
-+ 5% seeded noise: the benchmark's workload) and the 256^3 Taylor-Green box
-(U0 0.05, Re 1600) are stepped a few steps on the GPU in every precision
-mode and by the oracle in FP64 (and FP32), from the same initial state.
-The pipe run applies the reference's gauge (monitor, then the gauge
++ 5% seeded noise: the benchmark's workload), the same lattice with a rough
+wall (a mask that varies along z: the tiled kernel's variant for such masks,
+the plain kernel in FP64) and the 256^3 Taylor-Green box (U0 0.05, Re 1600)
+are stepped a few steps on the GPU in every precision mode and by the oracle
+in FP64 (and FP32), from the same initial state.
+The pipe runs apply the reference's gauge (monitor, then the gauge
 subtracted at the next step, FSLBM.cu:628-647) after step 2.
 
 Tolerances (after 4 steps):
@@ -36,8 +38,10 @@ STEPS = 4
 GAUGE_AFTER = 2
 # (velocity, pressure departure) bounds; measured after 4 steps:
 #   pipe FP16S 9.7e-6, 5.3e-4; BF16S 7.8e-5, 4.2e-3
+#   rough pipe FP16S 1.7e-5, 6.9e-4; BF16S 1.3e-4, 5.5e-3
 #   Taylor-Green FP16S 3.6e-4, 3.4e-3; BF16S 2.8e-3, 2.7e-2
 TOL16 = {("pipe", "f16s"): (3e-5, 2e-3), ("pipe", "bf16s"): (2.5e-4, 1.5e-2),
+         ("rough", "f16s"): (5e-5, 2e-3), ("rough", "bf16s"): (4e-4, 1.6e-2),
          ("taylor_green", "f16s"): (1e-3, 1e-2), ("taylor_green", "bf16s"): (8e-3, 8e-2)}
 
 
@@ -84,6 +88,20 @@ def pipe_case():
     return mask, st, prm
 
 
+def rough_case():
+    """The pipe's lattice with a rough wall (a mask that varies along z):
+    mean radius 120, corrugation of 4 cells and 600 seeded bumps."""
+    nx = ny = 256
+    nz, R, amp = 1024, 120.0, 4.0
+    re = 5300.0
+    nu = 0.05 * 2 * R / re
+    pin = inputs.blasius_pin(nu, nz, re, R)
+    mask, _ = inputs.rough_pipe_mask(nx, ny, nz, R, amp, 600, 5)
+    st = inputs.poiseuille_state(mask, R, re * nu / (2 * R), pin, 0.05, 3)
+    prm = RunParams(tblend=1.0, pin=pin, nu=nu, d1=2, delta=1, blend=0.0)
+    return mask, st, prm
+
+
 def tgv_case():
     n = 256
     nu = inputs.tgv_viscosity(n, 0.05, 1600.0)
@@ -117,9 +135,10 @@ def gpu_run(mask, st, prm, precision):
         return s.get_state()
 
 
-@pytest.fixture(scope="module", params=["pipe", "taylor_green"])
+@pytest.fixture(scope="module", params=["pipe", "rough", "taylor_green"])
 def case(request, orc):
-    mask, st, prm = pipe_case() if request.param == "pipe" else tgv_case()
+    mask, st, prm = {"pipe": pipe_case, "rough": rough_case,
+                     "taylor_green": tgv_case}[request.param]()
     ref64 = oracle_run(orc, mask, st, prm, np.float64)
     ref32 = oracle_run(orc, mask, st, prm, np.float32)
     return request.param, mask, st, prm, ref64, ref32
